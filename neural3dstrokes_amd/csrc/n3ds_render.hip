@@ -1,0 +1,918 @@
+// Fused stroke-field renderer: per-sample stroke SDF -> alpha/colour, ordered
+// "over" compositing and the volume-render ray march in one pass, with a
+// hand-written adjoint that recomputes the SDFs instead of storing per
+// (sample, stroke) terms.
+//
+// Semantics are those of the reference chain for a mixed stroke palette
+// (source/models.py:660-711 + source/utils/render.py:5-116): one
+// stroke_forward per stroke type placed into columns by stroke index,
+// compose_forward over all strokes (index S-1 in front), then
+// compute_alpha_weights / volumetric_rendering.
+//
+// Mapping (one wavefront per ray):
+//  * lane l owns samples l, l+64, ... (NS slots, n <= 64*NS);
+//  * strokes are visited in compositing order in chunks of 64: each lane
+//    tests one stroke against the ray (unit-space slab test for solids, the
+//    boxes of de Casteljau pieces for curves), the hits are compacted with a
+//    ballot and walked one by one with the stroke index broadcast by readlane,
+//    so the stroke record is fetched with scalar loads and the type switch is
+//    wave-uniform;
+//  * per-sample transmittance / density / colour live in registers; the ray
+//    march is a wavefront scan.
+// Culling is exact for the linear alpha transfer (alpha = 0 outside the
+// tested volume); for the Laplace transfer it drops alpha below `cull_eps`.
+#include "n3ds_capi.h"
+#include "n3ds_stroke_common.h"
+
+namespace n3ds {
+
+constexpr int REC = 64;     // floats per stroke record
+constexpr int NPIECE = 5;   // curve pieces bounded for culling
+constexpr int RWAVES = 4;   // wavefronts per block
+constexpr int GSLOTS = 32;  // canonical gradient slots per reduce round
+
+// record layout
+enum : int {
+    R_BASE = 0, R_FLAGS = 1, R_P = 2, R_NB = 3, R_OFFS = 4, R_OFFA = 5, R_OFFT = 6, R_CURVE = 7,
+    R_ROT = 8,      // 9: rotation entries r00..r22
+    R_TR = 17,      // 3: translation
+    R_SC = 20,      // 3: scale (single scale replicated)
+    R_ANG = 23,     // 3: euler angles
+    R_BP = 26,      // 2: base params
+    R_MINS = 28,    // min scale
+    R_E0 = 29,      // 3: unit box half extent
+    R_E1 = 32,      // 3: margin multiplier
+    R_CP = 8,       // 14: curve params (curves reuse the solid slots)
+    R_BOX = 22,     // NPIECE * 6: piece boxes
+    R_HULL = 52,    // 6: whole-curve box
+    R_RMAX = 58,    // max tube radius
+};
+
+struct RenderArgs {
+    const float* origins;
+    const float* dirs;
+    const float* viewdirs;
+    const float* radii;
+    const float* tdist;
+    const float* table;   // [S, REC]
+    const float* cparams; // [S, Q]
+    const float* dparams; // [S]
+    int64_t R;
+    int n, S, Q, Pmax, cid;
+    float delta;
+    int laplace, isr, cone, exact;
+    float coord_div;
+    float bg0, bg1, bg2;
+    float tail;  // culling margin multiplier
+};
+
+__device__ __forceinline__ int ibits(float f) { return __float_as_int(f); }
+__device__ __forceinline__ float fbits(int i) { return __int_as_float(i); }
+
+// ---------------------------------------------------------------------------
+// stroke table
+// ---------------------------------------------------------------------------
+__device__ void curve_piece_box(int base, const V3 K[4], float a, float b, float* box) {
+    // Bezier control points of the piece [a, b] (Hermite form), whose hull bounds it
+    V3 c0, c1, c2, c3;
+    if (base == QUAD_BEZIER) {
+        // quadratic P(t) = K0 (1-t)^2 + 2 K1 t(1-t) + K2 t^2
+        auto P = [&](float t) { float u = 1.f - t; return (u * u) * K[0] + (2.f * t * u) * K[1] + (t * t) * K[2]; };
+        auto dP = [&](float t) { return (2.f * (1.f - t)) * (K[1] - K[0]) + (2.f * t) * (K[2] - K[1]); };
+        c0 = P(a);
+        c3 = P(b);
+        c1 = c0 + ((b - a) * 0.5f) * dP(a);
+        c2 = c1;
+    } else {
+        bool power = base == CATMULL_ROM;
+        V3 pa = power ? curve_point<true>(a, K) : curve_point<false>(a, K);
+        V3 pb = power ? curve_point<true>(b, K) : curve_point<false>(b, K);
+        V3 ta = power ? curve_tangent<true>(a, K) : curve_tangent<false>(a, K);
+        V3 tb = power ? curve_tangent<true>(b, K) : curve_tangent<false>(b, K);
+        c0 = pa;
+        c3 = pb;
+        c1 = pa + ((b - a) / 3.f) * ta;
+        c2 = pb - ((b - a) / 3.f) * tb;
+    }
+    box[0] = fminf(fminf(c0.x, c1.x), fminf(c2.x, c3.x));
+    box[1] = fminf(fminf(c0.y, c1.y), fminf(c2.y, c3.y));
+    box[2] = fminf(fminf(c0.z, c1.z), fminf(c2.z, c3.z));
+    box[3] = fmaxf(fmaxf(c0.x, c1.x), fmaxf(c2.x, c3.x));
+    box[4] = fmaxf(fmaxf(c0.y, c1.y), fmaxf(c2.y, c3.y));
+    box[5] = fmaxf(fmaxf(c0.z, c1.z), fmaxf(c2.z, c3.z));
+}
+
+__global__ void render_prep_kernel(float* __restrict__ table, const int* __restrict__ ids,
+                                   const float* __restrict__ sp, int S, int Pmax, int* __restrict__ status) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S) return;
+    float* rec = table + (int64_t)k * REC;
+    for (int i = 0; i < REC; i++) rec[i] = 0.f;
+    uint32_t id = (uint32_t)ids[k];
+    int base = (int)(id >> 4);
+    uint32_t flags = id & 15u;
+    int P = num_shape_params(id);
+    if (base >= NUM_BASE || P > Pmax || ((flags & F_SS) && (flags & F_MS)) || (is_curve(base) && flags)) {
+        atomicOr(status, 1);
+        base = SPHERE; flags = 0; P = 0;
+    }
+    const float* row = sp + (int64_t)k * Pmax;
+    int nb = num_base_params(base);
+    rec[R_BASE] = fbits(base);
+    rec[R_FLAGS] = fbits((int)flags);
+    rec[R_P] = fbits(P);
+    rec[R_NB] = fbits(nb);
+    rec[R_CURVE] = fbits(is_curve(base) ? 1 : 0);
+    if (!is_curve(base)) {
+        Frame F = load_frame(row, P, flags);
+        rec[R_OFFS] = fbits(F.off_scale);
+        rec[R_OFFA] = fbits(F.off_angle);
+        rec[R_OFFT] = fbits(F.off_trans);
+        Rot Rm;
+        if (flags & F_R) Rm = F.R;
+        else for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) Rm.r[i][j] = (i == j) ? 1.f : 0.f;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) rec[R_ROT + 3 * i + j] = Rm.r[i][j];
+        rec[R_TR + 0] = F.trans.x; rec[R_TR + 1] = F.trans.y; rec[R_TR + 2] = F.trans.z;
+        rec[R_SC + 0] = F.scale.x; rec[R_SC + 1] = F.scale.y; rec[R_SC + 2] = F.scale.z;
+        rec[R_ANG + 0] = F.angle.x; rec[R_ANG + 1] = F.angle.y; rec[R_ANG + 2] = F.angle.z;
+        for (int i = 0; i < nb; i++) rec[R_BP + i] = row[i];
+        rec[R_MINS] = fminf(F.scale.x, fminf(F.scale.y, F.scale.z));
+        // unit-space box: points with sdf < m satisfy |p_a| < e0_a + e1_a m
+        V3 e0 = v3s(1.f), e1 = v3s(1.f);
+        float b0 = nb > 0 ? row[0] : 0.f, b1 = nb > 1 ? row[1] : 0.f;
+        switch (base) {
+            case ROUND_CUBE: e0 = v3s(1.f + b0); break;
+            case CAPPED_TORUS: e0 = v3(1.f + b1, 1.f + b1, b1); break;
+            case CAPSULE: e0 = v3(1.f, b0 + 1.f, 1.f); break;
+            case LINE: { float rr = 1.f + fabsf(b1); e0 = v3(rr, b0 + rr, rr); break; }
+            case TRIPRISM: e0 = v3(0.75f / TRI_C, b0, 1.f); e1 = v3(1.5f / TRI_C, 1.f, 2.f); break;
+            case OCTAHEDRON:
+            case TETRAHEDRON: e1 = v3s(1.7320509f); break;
+            default: break;
+        }
+        rec[R_E0 + 0] = e0.x * 1.0001f + 1e-5f; rec[R_E0 + 1] = e0.y * 1.0001f + 1e-5f; rec[R_E0 + 2] = e0.z * 1.0001f + 1e-5f;
+        rec[R_E1 + 0] = e1.x * 1.0001f; rec[R_E1 + 1] = e1.y * 1.0001f; rec[R_E1 + 2] = e1.z * 1.0001f;
+    } else {
+        rec[R_OFFS] = fbits(-1); rec[R_OFFA] = fbits(-1); rec[R_OFFT] = fbits(-1);
+        for (int i = 0; i < nb; i++) rec[R_CP + i] = row[i];
+        V3 K[4];
+        float r1, r2;
+        if (base == QUAD_BEZIER) {
+            K[0] = ld3(row); K[1] = ld3(row + 3); K[2] = ld3(row + 6); K[3] = K[2];
+            r1 = row[9]; r2 = row[10];
+        } else if (base == CUBIC_BEZIER) {
+            cubic_ctrl(row, K);
+            r1 = row[12]; r2 = row[13];
+        } else {
+            CatmullCoef c = catmull_coef(row);
+            for (int j = 0; j < 4; j++) K[j] = c.K[j];
+            r1 = row[12]; r2 = row[13];
+        }
+        float hull[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
+        for (int j = 0; j < NPIECE; j++) {
+            float* box = rec + R_BOX + 6 * j;
+            curve_piece_box(base, K, (float)j / NPIECE, (float)(j + 1) / NPIECE, box);
+            float pad = 1e-5f * (1.f + fmaxf(fmaxf(fabsf(box[0]), fabsf(box[3])), fmaxf(fabsf(box[2]), fabsf(box[5]))));
+            for (int a = 0; a < 3; a++) {
+                box[a] -= pad;
+                box[3 + a] += pad;
+                hull[a] = fminf(hull[a], box[a]);
+                hull[3 + a] = fmaxf(hull[3 + a], box[3 + a]);
+            }
+        }
+        for (int a = 0; a < 6; a++) rec[R_HULL + a] = hull[a];
+        rec[R_RMAX] = fmaxf(fmaxf(r1, r2), 0.f) * 1.0001f + 1e-6f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// culling: parametric interval of the ray segment inside the stroke's bound
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void slab(float o, float d, float lo, float hi, float& t0, float& t1) {
+    float inv = 1.f / d;
+    float a = (lo - o) * inv, b = (hi - o) * inv;
+    t0 = fmaxf(t0, fminf(a, b));
+    t1 = fminf(t1, fmaxf(a, b));
+}
+
+__device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o, V3 d, float m, int isr, float tlo,
+                                            float thi, float& ta, float& tb) {
+    float t0 = tlo, t1 = thi;
+    if (ibits(rec[R_CURVE])) {
+        float e = rec[R_RMAX] + m;
+        float h0 = tlo, h1 = thi;
+        slab(o.x, d.x, rec[R_HULL + 0] - e, rec[R_HULL + 3] + e, h0, h1);
+        slab(o.y, d.y, rec[R_HULL + 1] - e, rec[R_HULL + 4] + e, h0, h1);
+        slab(o.z, d.z, rec[R_HULL + 2] - e, rec[R_HULL + 5] + e, h0, h1);
+        if (!(h0 <= h1)) return false;
+        float u0 = 1e30f, u1 = -1e30f;
+#pragma unroll
+        for (int j = 0; j < NPIECE; j++) {
+            const float* bx = rec + R_BOX + 6 * j;
+            float p0 = tlo, p1 = thi;
+            slab(o.x, d.x, bx[0] - e, bx[3] + e, p0, p1);
+            slab(o.y, d.y, bx[1] - e, bx[4] + e, p0, p1);
+            slab(o.z, d.z, bx[2] - e, bx[5] + e, p0, p1);
+            if (p0 <= p1) { u0 = fminf(u0, p0); u1 = fmaxf(u1, p1); }
+        }
+        t0 = u0;
+        t1 = u1;
+    } else {
+        const float* r = rec + R_ROT;
+        V3 tr = ld3(rec + R_TR), sc = ld3(rec + R_SC);
+        V3 q = o - tr;
+        V3 A = v3((r[0] * q.x + r[3] * q.y + r[6] * q.z) / sc.x, (r[1] * q.x + r[4] * q.y + r[7] * q.z) / sc.y,
+                  (r[2] * q.x + r[5] * q.y + r[8] * q.z) / sc.z);
+        V3 B = v3((r[0] * d.x + r[3] * d.y + r[6] * d.z) / sc.x, (r[1] * d.x + r[4] * d.y + r[7] * d.z) / sc.y,
+                  (r[2] * d.x + r[5] * d.y + r[8] * d.z) / sc.z);
+        float mu = isr ? m / rec[R_MINS] : m;
+        float ex = rec[R_E0 + 0] + rec[R_E1 + 0] * mu;
+        float ey = rec[R_E0 + 1] + rec[R_E1 + 1] * mu;
+        float ez = rec[R_E0 + 2] + rec[R_E1 + 2] * mu;
+        slab(A.x, B.x, -ex, ex, t0, t1);
+        slab(A.y, B.y, -ey, ey, t0, t1);
+        slab(A.z, B.z, -ez, ez, t0, t1);
+    }
+    if (!(t0 <= t1)) return false;
+    float pad = 1e-4f * (t1 - t0) + 1e-5f * (fabsf(t0) + fabsf(t1)) + 1e-6f;
+    ta = t0 - pad;
+    tb = t1 + pad;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// per (sample, stroke) evaluation
+// ---------------------------------------------------------------------------
+struct Pair {
+    V3 p;
+    float sdf, alpha, sc, fac;
+};
+
+__device__ __forceinline__ Pair eval_pair(const float* __restrict__ rec, int base, bool curve, V3 x, float r,
+                                          const RenderArgs& A) {
+    Pair q;
+    const float* prm = curve ? rec + R_CP : rec + R_BP;
+    if (curve) {
+        q.p = x;
+    } else {
+        const float* m = rec + R_ROT;
+        V3 d = x - ld3(rec + R_TR);
+        V3 pr = v3(m[0] * d.x + m[3] * d.y + m[6] * d.z, m[1] * d.x + m[4] * d.y + m[7] * d.z,
+                   m[2] * d.x + m[5] * d.y + m[8] * d.z);
+        V3 s = ld3(rec + R_SC);
+        q.p = v3(pr.x / s.x, pr.y / s.y, pr.z / s.z);
+    }
+    q.sdf = sdf_eval(base, q.p, prm);
+    q.fac = 1.f;
+    if (A.isr && !curve) {
+        uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
+        if (flags & F_SS) { q.fac = rec[R_SC]; r = r / q.fac; }
+        else if (flags & F_MS) { q.fac = radius_factor(ref_dir(base, q.p, prm), ld3(rec + R_SC)); r = r / q.fac; }
+    }
+    q.sc = (A.laplace ? 2.f : 0.5f) / (A.delta * r);
+    q.alpha = alpha_from_sdf(q.sdf, q.sc, A.delta, A.laplace != 0);
+    return q;
+}
+
+template <int CID>
+__device__ __forceinline__ V3 stroke_color_uniform(const float* __restrict__ cp, V3 vdir) {
+    if (CID == 0) return ld3(cp);
+    if (CID >= 2) return color_eval_t<CID>(v3s(0.f), vdir, cp);
+    return v3s(0.f);
+}
+
+template <int NS>
+struct RayState {
+    V3 x[NS];
+    float r[NS];
+    float t[NS];
+    bool valid[NS];
+};
+
+template <int NS>
+__device__ __forceinline__ void load_ray(const RenderArgs& A, int64_t ray, int lane, RayState<NS>& rs, V3& o, V3& d,
+                                         V3& vd, float& tlo, float& thi, float& rmax) {
+    o = ld3(A.origins + 3 * ray);
+    d = ld3(A.dirs + 3 * ray);
+    vd = ld3(A.viewdirs + 3 * ray);
+    const float radii = A.radii[ray];
+    const float* td = A.tdist + ray * (A.n + 1);
+    tlo = 0.5f * (td[0] + td[1]);
+    thi = 0.5f * (td[A.n - 1] + td[A.n]);
+    rmax = A.cone ? radii * fmaxf(fabsf(tlo), fabsf(thi)) : radii;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + 64 * s;
+        rs.valid[s] = i < A.n;
+        int ii = rs.valid[s] ? i : 0;
+        float t = (td[ii] + td[ii + 1]) / 2.f;
+        rs.t[s] = t;
+        V3 xw = d * t + o;
+        if (A.coord_div != 1.f) xw = v3(xw.x / A.coord_div, xw.y / A.coord_div, xw.z / A.coord_div);
+        rs.x[s] = xw;
+        rs.r[s] = A.cone ? radii * t : radii;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+template <int NS, int CID>
+__global__ __launch_bounds__(64 * RWAVES) void render_fwd_kernel(RenderArgs A, float* __restrict__ rgb_out,
+                                                                 float* __restrict__ acc_out,
+                                                                 float* __restrict__ depth_out,
+                                                                 float* __restrict__ state,
+                                                                 unsigned long long* __restrict__ counters) {
+    __shared__ unsigned long long cnt[2 * NUM_BASE + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (counters) {
+        for (int i = threadIdx.x; i < 2 * NUM_BASE + 1; i += blockDim.x) cnt[i] = 0ull;
+        __syncthreads();
+    }
+    const int64_t ray = (int64_t)blockIdx.x * RWAVES + wave;
+    if (ray < A.R) {
+        RayState<NS> rs;
+        V3 o, d, vd;
+        float tlo, thi, rmax;
+        load_ray<NS>(A, ray, lane, rs, o, d, vd, tlo, thi, rmax);
+        const float m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
+        V3 oc = o, dc = d;
+        if (A.coord_div != 1.f) {
+            oc = v3(o.x / A.coord_div, o.y / A.coord_div, o.z / A.coord_div);
+            dc = v3(d.x / A.coord_div, d.y / A.coord_div, d.z / A.coord_div);
+        }
+        float T[NS], Dn[NS];
+        V3 C[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) { T[s] = 1.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
+        for (int base0 = 0; base0 < A.S; base0 += 64) {
+            int k = A.S - 1 - (base0 + lane);
+            float ta = 0.f, tb = -1.f;
+            bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, oc, dc, m, A.isr, tlo, thi, ta, tb);
+            unsigned long long mask = __ballot(hit);
+            while (mask) {
+                int j = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int ks = __builtin_amdgcn_readlane(k, j);
+                const float sa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ta), j));
+                const float sb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tb), j));
+                const float* rec = A.table + (int64_t)ks * REC;
+                const int base = ibits(rec[R_BASE]);
+                const bool curve = ibits(rec[R_CURVE]) != 0;
+                const float* cp = A.cparams + (int64_t)ks * A.Q;
+                const float dk = A.dparams[ks];
+                const V3 cu = stroke_color_uniform<CID>(cp, vd);
+                bool fpos[NS], fev[NS];
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    bool act = rs.valid[s] && rs.t[s] >= sa && rs.t[s] <= sb;
+                    fpos[s] = false;
+                    fev[s] = act;
+                    if (act) {
+                        Pair q = eval_pair(rec, base, curve, rs.x[s], rs.r[s], A);
+                        if (q.alpha != 0.f) {
+                            V3 c = (CID == 1) ? color_eval_t<1>(q.p, vd, cp) : cu;
+                            float w = q.alpha * T[s];
+                            T[s] *= (1.f - q.alpha);
+                            Dn[s] += dk * w;
+                            C[s] = C[s] + c * w;
+                        }
+                        fpos[s] = q.alpha > 0.f;
+                    }
+                }
+                if (counters) {
+                    int tp = 0, te = 0;
+#pragma unroll
+                    for (int s = 0; s < NS; s++) {
+                        tp += __popcll(__ballot(fpos[s]));
+                        te += __popcll(__ballot(fev[s]));
+                    }
+                    if (lane == 0) {
+                        atomicAdd(&cnt[base], (unsigned long long)tp);
+                        atomicAdd(&cnt[NUM_BASE + base], (unsigned long long)te);
+                        atomicAdd(&cnt[2 * NUM_BASE], 1ull);
+                    }
+                }
+            }
+        }
+        // save per-sample composite state and run the ray march
+        const int64_t N = A.R * (int64_t)A.n;
+        const float* td = A.tdist + ray * (A.n + 1);
+        const float dn = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+        float sd[NS], rgbn[NS][3], wgt[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int i = lane + 64 * s;
+            float scl = 1.f / (1.f + 1e-6f - T[s]);
+            rgbn[s][0] = clampf(C[s].x * scl, 0.f, 1.f);
+            rgbn[s][1] = clampf(C[s].y * scl, 0.f, 1.f);
+            rgbn[s][2] = clampf(C[s].z * scl, 0.f, 1.f);
+            if (rs.valid[s]) {
+                int64_t si = ray * A.n + i;
+                state[si] = Dn[s];
+                state[N + si] = C[s].x;
+                state[2 * N + si] = C[s].y;
+                state[3 * N + si] = C[s].z;
+                state[4 * N + si] = T[s];
+                sd[s] = Dn[s] * ((td[i + 1] - td[i]) * dn);
+            } else {
+                sd[s] = 0.f;
+            }
+        }
+        // exclusive prefix sum of density*delta over samples (slot-major order)
+        float carry = 0.f;
+        float acc = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, dsum = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            float v = sd[s];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                float u = __shfl_up(v, off, 64);
+                if (lane >= off) v += u;
+            }
+            float excl = carry + v - sd[s];
+            float trans = expf(-excl);
+            float al = 1.f - expf(-sd[s]);
+            float w = rs.valid[s] ? al * trans : 0.f;
+            wgt[s] = w;
+            carry += __shfl(v, 63, 64);
+            acc += w;
+            r0 += w * rgbn[s][0];
+            r1 += w * rgbn[s][1];
+            r2 += w * rgbn[s][2];
+            dsum += w * rs.t[s];
+        }
+        acc = wave_sum(acc);
+        r0 = wave_sum(r0);
+        r1 = wave_sum(r1);
+        r2 = wave_sum(r2);
+        dsum = wave_sum(dsum);
+        if (lane == 0) {
+            float bgw = fmaxf(1.f - acc, 0.f);
+            rgb_out[3 * ray + 0] = r0 + bgw * A.bg0;
+            rgb_out[3 * ray + 1] = r1 + bgw * A.bg1;
+            rgb_out[3 * ray + 2] = r2 + bgw * A.bg2;
+            acc_out[ray] = acc;
+            float dep = dsum / fmaxf(acc, 1.1920929e-07f);
+            if (isnan(dep)) dep = INFINITY;
+            depth_out[ray] = fminf(fmaxf(dep, td[0]), td[A.n]);
+        }
+    }
+    if (counters) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 2 * NUM_BASE + 1; i += blockDim.x)
+            if (cnt[i]) atomicAdd(counters + i, cnt[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward
+// ---------------------------------------------------------------------------
+// Reduce-scatter of 32 per-lane values v[OFF .. OFF+32) across the wavefront:
+// afterwards lane l holds the wavefront sum of value OFF + (l >> 1).  Each step
+// exchanges half of the remaining values with the partner lane, so 32 values
+// cost 16+8+4+2+1+1 shuffles instead of 32*6.  Destroys v[OFF..OFF+16).
+template <int OFF, int N>
+__device__ __forceinline__ float reduce_scatter32(float (&v)[N], int lane) {
+#pragma unroll
+    for (int step = 0; step < 5; step++) {
+        const int half = 16 >> step;           // values kept after this step
+        const int mask = 32 >> step;           // partner lane distance
+        const bool upper = (lane & mask) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            float keep = upper ? v[OFF + half + i] : v[OFF + i];
+            float send = upper ? v[OFF + i] : v[OFF + half + i];
+            v[OFF + i] = keep + __shfl_xor(send, mask, 64);
+        }
+    }
+    return v[OFF] + __shfl_xor(v[OFF], 1, 64);
+}
+
+// canonical gradient slot -> column in the [Pmax | Q | 1] accumulator row (-1: none)
+__device__ __forceinline__ int slot_column(int slot, const float* __restrict__ rec, int Pmax, int Q) {
+    const bool curve = ibits(rec[R_CURVE]) != 0;
+    const int nb = ibits(rec[R_NB]);
+    if (slot >= 14) {
+        int c = slot - 14;
+        if (c < Q) return Pmax + c;
+        if (c == Q) return Pmax + Q;
+        return -1;
+    }
+    if (curve) return slot < nb ? slot : -1;
+    if (slot < 2) return slot < nb ? slot : -1;
+    const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
+    if (slot < 5) {
+        int off = ibits(rec[R_OFFS]);
+        if (off < 0) return -1;
+        if (flags & F_SS) return slot == 2 ? off : -1;
+        return off + slot - 2;
+    }
+    if (slot < 8) {
+        int off = ibits(rec[R_OFFA]);
+        return off < 0 ? -1 : off + slot - 5;
+    }
+    if (slot < 11) {
+        int off = ibits(rec[R_OFFT]);
+        return off < 0 ? -1 : off + slot - 8;
+    }
+    return -1;
+}
+
+template <int NS, int CID>
+__global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, const float* __restrict__ g_rgb,
+                                                                 const float* __restrict__ g_acc,
+                                                                 const float* __restrict__ g_depth,
+                                                                 const float* __restrict__ state,
+                                                                 float* __restrict__ partials, int use_lds,
+                                                                 float* __restrict__ gdirect) {
+    extern __shared__ float gacc[];  // [S * PG] when use_lds
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int PG = A.Pmax + A.Q + 1;
+    const int64_t nacc = (int64_t)A.S * PG;
+    if (use_lds) {
+        for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) gacc[i] = 0.f;
+        __syncthreads();
+    }
+    const int64_t N = A.R * (int64_t)A.n;
+    const int64_t nwaves = (int64_t)gridDim.x * RWAVES;
+    for (int64_t ray = (int64_t)blockIdx.x * RWAVES + wave; ray < A.R; ray += nwaves) {
+        RayState<NS> rs;
+        V3 o, d, vd;
+        float tlo, thi, rmax;
+        load_ray<NS>(A, ray, lane, rs, o, d, vd, tlo, thi, rmax);
+        const float m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
+        V3 oc = o, dc = d;
+        if (A.coord_div != 1.f) {
+            oc = v3(o.x / A.coord_div, o.y / A.coord_div, o.z / A.coord_div);
+            dc = v3(d.x / A.coord_div, d.y / A.coord_div, d.z / A.coord_div);
+        }
+        const float* td = A.tdist + ray * (A.n + 1);
+        const float dn = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+        // ---- ray-march adjoint from the saved per-sample state
+        float Dt[NS], Tn[NS], sdl[NS], delta_s[NS];
+        V3 Ct[NS], cn[NS];
+        bool inr[NS][3];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int i = lane + 64 * s;
+            if (rs.valid[s]) {
+                int64_t si = ray * A.n + i;
+                Dt[s] = state[si];
+                Ct[s] = v3(state[N + si], state[2 * N + si], state[3 * N + si]);
+                Tn[s] = state[4 * N + si];
+                delta_s[s] = (td[i + 1] - td[i]) * dn;
+            } else {
+                Dt[s] = 0.f; Ct[s] = v3s(0.f); Tn[s] = 1.f; delta_s[s] = 0.f;
+            }
+            float scl = 1.f / (1.f + 1e-6f - Tn[s]);
+            V3 sv = Ct[s] * scl;
+            inr[s][0] = 0.f <= sv.x && sv.x <= 1.f;
+            inr[s][1] = 0.f <= sv.y && sv.y <= 1.f;
+            inr[s][2] = 0.f <= sv.z && sv.z <= 1.f;
+            cn[s] = v3(clampf(sv.x, 0.f, 1.f), clampf(sv.y, 0.f, 1.f), clampf(sv.z, 0.f, 1.f));
+            sdl[s] = Dt[s] * delta_s[s];
+        }
+        float wv[NS], trv[NS];
+        float carry = 0.f, acc = 0.f, dsum = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            float v = sdl[s];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                float u = __shfl_up(v, off, 64);
+                if (lane >= off) v += u;
+            }
+            float excl = carry + v - sdl[s];
+            trv[s] = expf(-excl);
+            float al = 1.f - expf(-sdl[s]);
+            wv[s] = rs.valid[s] ? al * trv[s] : 0.f;
+            carry += __shfl(v, 63, 64);
+            acc += wv[s];
+            dsum += wv[s] * rs.t[s];
+        }
+        acc = wave_sum(acc);
+        dsum = wave_sum(dsum);
+        const V3 grgb = ld3(g_rgb + 3 * ray);
+        const float gacc_r = g_acc ? g_acc[ray] : 0.f;
+        const float gdep = g_depth ? g_depth[ray] : 0.f;
+        const float bgdot = (1.f - acc >= 0.f) ? grgb.x * A.bg0 + grgb.y * A.bg1 + grgb.z * A.bg2 : 0.f;
+        const float accc = fmaxf(acc, 1.1920929e-07f);
+        const float dep = dsum / accc;
+        const bool dep_in = dep >= td[0] && dep <= td[A.n];
+        float gw[NS], gsig[NS];
+        float suf_in = 0.f;  // sum over later samples of w_j gw_j
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            gw[s] = dot(grgb, cn[s]) + gacc_r - bgdot;
+            if (dep_in) gw[s] += gdep * (acc > 1.1920929e-07f ? (rs.t[s] - dep) / accc : rs.t[s] / accc);
+            if (!rs.valid[s]) gw[s] = 0.f;
+        }
+        // suffix (exclusive) sums of w*gw over samples after this one
+#pragma unroll
+        for (int s = NS - 1; s >= 0; s--) {
+            float v = wv[s] * gw[s];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                float u = __shfl_down(v, off, 64);
+                if (lane + off < 64) v += u;
+            }
+            float excl = suf_in + v - wv[s] * gw[s];
+            float tnext = trv[s] * expf(-sdl[s]);
+            gsig[s] = delta_s[s] * (tnext * gw[s] - excl);
+            suf_in += __shfl(v, 0, 64);
+        }
+        // per-sample compositing adjoint inputs
+        float gD[NS], O[NS], Tp[NS], D2[NS];
+        V3 gC[NS], C2[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            gD[s] = rs.valid[s] ? gsig[s] : 0.f;
+            V3 gcv = grgb * wv[s];
+            gC[s] = v3(inr[s][0] ? gcv.x : 0.f, inr[s][1] ? gcv.y : 0.f, inr[s][2] ? gcv.z : 0.f);
+            O[s] = 1.f + 1e-6f - Tn[s];
+            Tp[s] = 1.f;
+            D2[s] = 0.f;
+            C2[s] = v3s(0.f);
+        }
+        // ---- stroke loop in compositing order
+        for (int base0 = 0; base0 < A.S; base0 += 64) {
+            int k = A.S - 1 - (base0 + lane);
+            float ta = 0.f, tb = -1.f;
+            bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, oc, dc, m, A.isr, tlo, thi, ta, tb);
+            unsigned long long mask = __ballot(hit);
+            while (mask) {
+                int j = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int ks = __builtin_amdgcn_readlane(k, j);
+                const float sa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ta), j));
+                const float sb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tb), j));
+                const float* rec = A.table + (int64_t)ks * REC;
+                const int base = ibits(rec[R_BASE]);
+                const bool curve = ibits(rec[R_CURVE]) != 0;
+                const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
+                const float* cp = A.cparams + (int64_t)ks * A.Q;
+                const float dk = A.dparams[ks];
+                const V3 cu = stroke_color_uniform<CID>(cp, vd);
+                // canonical slots: 0..13 shape, 14..14+Q-1 colour, 14+Q density
+                constexpr int NQ = num_color_params(CID);
+                constexpr int GTOT = (15 + NQ) <= GSLOTS ? GSLOTS : 2 * GSLOTS;
+                float G[GTOT];
+#pragma unroll
+                for (int g = 0; g < GTOT; g++) G[g] = 0.f;
+                bool any = false;
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    bool act = rs.valid[s] && rs.t[s] >= sa && rs.t[s] <= sb;
+                    if (!act) continue;
+                    Pair q = eval_pair(rec, base, curve, rs.x[s], rs.r[s], A);
+                    const float a = q.alpha;
+                    if (a == 0.f) continue;
+                    any = true;
+                    V3 c = (CID == 1) ? color_eval_t<1>(q.p, vd, cp) : cu;
+                    const float Tk = Tp[s];
+                    const float w = a * Tk;
+                    const float Ta = Tk * (1.f - a);
+                    D2[s] += dk * w;
+                    C2[s] = C2[s] + c * w;
+                    const float sufD = Dt[s] - D2[s];
+                    const V3 sufC = Ct[s] - C2[s];
+                    const float inv1 = 1.f / fmaxf(1.f - a, 1e-6f);
+                    const float iO = 1.f / O[s];
+                    float ga;
+                    if (A.exact) {
+                        ga = gD[s] * (dk * Tk - sufD * inv1);
+                        ga += gC[s].x * ((c.x * Tk - sufC.x * inv1) * iO - Ct[s].x * (Tn[s] * inv1) * iO * iO);
+                        ga += gC[s].y * ((c.y * Tk - sufC.y * inv1) * iO - Ct[s].y * (Tn[s] * inv1) * iO * iO);
+                        ga += gC[s].z * ((c.z * Tk - sufC.z * inv1) * iO - Ct[s].z * (Tn[s] * inv1) * iO * iO);
+                    } else {
+                        ga = gD[s] * (Ta * dk - sufD) * inv1;
+                        float s2 = inv1 * iO * iO;
+                        ga += gC[s].x * (Ta * c.x * (O[s] - a) - sufC.x) * s2;
+                        ga += gC[s].y * (Ta * c.y * (O[s] - a) - sufC.y) * s2;
+                        ga += gC[s].z * (Ta * c.z * (O[s] - a) - sufC.z) * s2;
+                    }
+                    Tp[s] = Ta;
+                    // colour and density parameter gradients
+                    V3 gcol = gC[s] * (w * iO);
+                    V3 gpos = color_grad_t<CID>(q.p, vd, cp, gcol, G + 14, A.exact != 0);
+                    G[14 + NQ] += gD[s] * w;
+                    // shape gradients through alpha
+                    const float gs = ga * dalpha_dsdf(q.sdf, a, q.sc, A.delta, A.laplace != 0, A.exact != 0);
+                    const float* prm = curve ? rec + R_CP : rec + R_BP;
+                    V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
+                    gpos = gpos + dsdp * gs;
+                    if (!curve) {
+                        float gs1 = 0.f;
+                        V3 gs3 = v3s(0.f), gang = v3s(0.f), gtr = v3s(0.f);
+                        Frame F;
+                        F.trans = ld3(rec + R_TR);
+                        F.angle = ld3(rec + R_ANG);
+                        F.scale = ld3(rec + R_SC);
+#pragma unroll
+                        for (int ii = 0; ii < 3; ii++)
+#pragma unroll
+                            for (int jj = 0; jj < 3; jj++) F.R.r[ii][jj] = rec[R_ROT + 3 * ii + jj];
+                        if (A.exact && A.isr && (flags & (F_SS | F_MS))) {
+                            float gsc = ga * dalpha_dscale(q.sdf, a, q.sc, A.delta, A.laplace != 0);
+                            if (flags & F_SS) gs1 += gsc * q.sc / F.scale.x;
+                            else radius_factor_adjoint(base, q.p, prm, F.scale, gsc * q.sc / q.fac, gpos, gs3, G);
+                        }
+                        xform_adjoint(gpos, q.p, rs.x[s], F, flags, gs1, gs3, gang, gtr);
+                        if (flags & F_SS) G[2] += gs1;
+                        else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
+                        G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
+                        G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
+                    }
+                }
+                // wave reduction and accumulation
+                if (__ballot(any) == 0ull) continue;
+                float* dst = use_lds ? gacc : gdirect;
+                float v = reduce_scatter32<0>(G, lane);
+                if ((lane & 1) == 0) {
+                    int col = slot_column(lane >> 1, rec, A.Pmax, A.Q);
+                    if (col >= 0 && v != 0.f) atomicAdd(&dst[(int64_t)ks * PG + col], v);
+                }
+                if (GTOT > GSLOTS) {
+                    float v2 = reduce_scatter32<GTOT - GSLOTS>(G, lane);
+                    if ((lane & 1) == 0) {
+                        int col = slot_column(GSLOTS + (lane >> 1), rec, A.Pmax, A.Q);
+                        if (col >= 0 && v2 != 0.f) atomicAdd(&dst[(int64_t)ks * PG + col], v2);
+                    }
+                }
+            }
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        float* out = partials + (int64_t)blockIdx.x * nacc;
+        for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) out[i] = gacc[i];
+    }
+}
+
+// sum per-block partial rows and scatter into the parameter-gradient tensors (+=)
+__global__ void render_reduce_kernel(const float* __restrict__ partials, int nparts, int S, int Pmax, int Q,
+                                     float* __restrict__ gshape, float* __restrict__ gcolor,
+                                     float* __restrict__ gdens) {
+    const int PG = Pmax + Q + 1;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)S * PG) return;
+    float s = 0.f;
+    for (int b = 0; b < nparts; b++) s += partials[(int64_t)b * S * PG + i];
+    int k = (int)(i / PG), c = (int)(i - (int64_t)k * PG);
+    if (c < Pmax) gshape[(int64_t)k * Pmax + c] += s;
+    else if (c < Pmax + Q) gcolor[(int64_t)k * Q + (c - Pmax)] += s;
+    else gdens[k] += s;
+}
+
+static int g_num_cu = 0;
+static int g_max_lds = 0;
+
+static int device_info() {
+    if (g_num_cu == 0) {
+        int dev;
+        N3DS_HIP_CHECK(hipGetDevice(&dev));
+        N3DS_HIP_CHECK(hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        N3DS_HIP_CHECK(hipDeviceGetAttribute(&g_max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    }
+    return N3DS_OK;
+}
+
+}  // namespace n3ds
+
+using namespace n3ds;
+
+static int render_check(int64_t R, int n, int S, int Pmax, int Q, int cid) {
+    N3DS_REQUIRE(R >= 0 && S >= 0, "negative size");
+    N3DS_REQUIRE(n >= 2 && n <= 256, "samples per ray must be in [2, 256] (got %d)", n);
+    N3DS_REQUIRE(Pmax >= 1 && Pmax <= 23, "shape_params width %d out of range", Pmax);
+    N3DS_REQUIRE(cid >= 0 && cid < 4, "color_id %d out of range", cid);
+    N3DS_REQUIRE(Q == num_color_params(cid), "color_params width %d does not match color_id %d", Q, cid);
+    return N3DS_OK;
+}
+
+static RenderArgs make_args(const float* origins, const float* dirs, const float* viewdirs, const float* radii,
+                            const float* tdist, const float* table, const float* cparams, const float* dparams,
+                            int64_t R, int n, int S, int Pmax, int Q, int cid, float delta, int laplace, int isr,
+                            int cone, float coord_div, const float* bg, float cull_eps, int exact) {
+    RenderArgs A;
+    A.origins = origins; A.dirs = dirs; A.viewdirs = viewdirs; A.radii = radii; A.tdist = tdist; A.table = table;
+    A.cparams = cparams; A.dparams = dparams; A.R = R; A.n = n; A.S = S; A.Q = Q; A.Pmax = Pmax; A.cid = cid;
+    A.delta = delta; A.laplace = laplace; A.isr = isr; A.cone = cone; A.exact = exact;
+    A.coord_div = coord_div; A.bg0 = bg[0]; A.bg1 = bg[1]; A.bg2 = bg[2];
+    if (laplace) {
+        float eps = cull_eps > 0.f ? cull_eps : 1e-7f;
+        A.tail = fmaxf(1.f, 0.5f * logf(0.5f / eps)) * 1.001f;
+    } else {
+        A.tail = 1.001f;
+    }
+    return A;
+}
+
+N3DS_EXPORT int64_t n3ds_render_table_floats(int n_strokes) { return (int64_t)n_strokes * REC; }
+
+N3DS_EXPORT int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* shape_params,
+                                    int n_strokes, int shape_width, int* status, void* stream) {
+    N3DS_REQUIRE(n_strokes >= 0 && shape_width >= 1 && shape_width <= 23, "bad stroke table sizes");
+    if (n_strokes == 0) return N3DS_OK;
+    hipLaunchKernelGGL(render_prep_kernel, dim3((n_strokes + 127) / 128), dim3(128), 0, (hipStream_t)stream, table,
+                       stroke_ids, shape_params, n_strokes, shape_width, status);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* directions, const float* viewdirs,
+                                    const float* radii, const float* tdist, const float* table,
+                                    const float* color_params, const float* density_params, int64_t n_rays,
+                                    int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
+                                    float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
+                                    float coord_div, const float* bg_color, float cull_eps, float* rgb_out,
+                                    float* acc_out, float* depth_out, float* state_out,
+                                    unsigned long long* counters, void* stream) {
+    int st = render_check(n_rays, n_samples, n_strokes, shape_width, color_width, color_id);
+    if (st) return st;
+    if (n_rays == 0) return N3DS_OK;
+    RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
+                             n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
+                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps, 0);
+    dim3 grid((unsigned)((n_rays + RWAVES - 1) / RWAVES)), block(64 * RWAVES);
+    hipStream_t s = (hipStream_t)stream;
+    int ns = (n_samples + 63) / 64;
+    typedef void (*Fwd)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
+    static const Fwd tab[4][4] = {
+        {render_fwd_kernel<1, 0>, render_fwd_kernel<1, 1>, render_fwd_kernel<1, 2>, render_fwd_kernel<1, 3>},
+        {render_fwd_kernel<2, 0>, render_fwd_kernel<2, 1>, render_fwd_kernel<2, 2>, render_fwd_kernel<2, 3>},
+        {render_fwd_kernel<3, 0>, render_fwd_kernel<3, 1>, render_fwd_kernel<3, 2>, render_fwd_kernel<3, 3>},
+        {render_fwd_kernel<4, 0>, render_fwd_kernel<4, 1>, render_fwd_kernel<4, 2>, render_fwd_kernel<4, 3>}};
+    hipLaunchKernelGGL(tab[ns - 1][color_id], grid, block, 0, s, A, rgb_out, acc_out, depth_out, state_out, counters);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+// Number of floats of device workspace the backward needs (per-block partial gradient rows).
+N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_strokes, int shape_width,
+                                                          int color_width) {
+    if (device_info()) return -1;
+    int64_t PG = shape_width + color_width + 1;
+    int64_t lds = (int64_t)n_strokes * PG * 4;
+    if (lds > g_max_lds) return (int64_t)n_strokes * PG;  // global-atomic path
+    int64_t nb = (int64_t)g_num_cu * 2;
+    int64_t need = (n_rays + RWAVES - 1) / RWAVES;
+    if (nb > need) nb = need;
+    if (nb < 1) nb = 1;
+    return nb * n_strokes * PG;
+}
+
+N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directions, const float* viewdirs,
+                                     const float* radii, const float* tdist, const float* table,
+                                     const float* color_params, const float* density_params, int64_t n_rays,
+                                     int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
+                                     float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
+                                     float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
+                                     const float* grad_rgb, const float* grad_acc, const float* grad_depth,
+                                     const float* state, float* workspace, int64_t workspace_floats,
+                                     float* grad_shape, float* grad_color, float* grad_density, void* stream) {
+    int st = render_check(n_rays, n_samples, n_strokes, shape_width, color_width, color_id);
+    if (st) return st;
+    st = device_info();
+    if (st) return st;
+    if (n_rays == 0 || n_strokes == 0) return N3DS_OK;
+    RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
+                             n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
+                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                             adjoint_exact);
+    const int64_t PG = shape_width + color_width + 1;
+    const int64_t lds = (int64_t)n_strokes * PG * 4;
+    const int use_lds = lds <= g_max_lds;
+    int64_t nb = (int64_t)g_num_cu * 2;
+    int64_t need = (n_rays + RWAVES - 1) / RWAVES;
+    if (nb > need) nb = need;
+    if (nb < 1) nb = 1;
+    const int64_t ws_need = use_lds ? nb * n_strokes * PG : (int64_t)n_strokes * PG;
+    N3DS_REQUIRE(workspace_floats >= ws_need, "workspace has %lld floats, need %lld", (long long)workspace_floats,
+                 (long long)ws_need);
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_lds) N3DS_HIP_CHECK(hipMemsetAsync(workspace, 0, sizeof(float) * ws_need, s));
+    dim3 grid((unsigned)nb), block(64 * RWAVES);
+    size_t shm = use_lds ? (size_t)lds : 0;
+    float* parts = use_lds ? workspace : nullptr;
+    float* direct = use_lds ? nullptr : workspace;
+    int ns = (n_samples + 63) / 64;
+    typedef void (*Bwd)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
+    static const Bwd tab[4][4] = {
+        {render_bwd_kernel<1, 0>, render_bwd_kernel<1, 1>, render_bwd_kernel<1, 2>, render_bwd_kernel<1, 3>},
+        {render_bwd_kernel<2, 0>, render_bwd_kernel<2, 1>, render_bwd_kernel<2, 2>, render_bwd_kernel<2, 3>},
+        {render_bwd_kernel<3, 0>, render_bwd_kernel<3, 1>, render_bwd_kernel<3, 2>, render_bwd_kernel<3, 3>},
+        {render_bwd_kernel<4, 0>, render_bwd_kernel<4, 1>, render_bwd_kernel<4, 2>, render_bwd_kernel<4, 3>}};
+    if (shm > 65536) N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)tab[ns - 1][color_id],
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(tab[ns - 1][color_id], grid, block, shm, s, A, grad_rgb, grad_acc, grad_depth, state, parts,
+                       use_lds, direct);
+    N3DS_LAUNCH_CHECK();
+    int64_t tot = (int64_t)n_strokes * PG;
+    hipLaunchKernelGGL(render_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace,
+                       use_lds ? (int)nb : 1, n_strokes, shape_width, color_width, grad_shape, grad_color,
+                       grad_density);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}

@@ -1,0 +1,128 @@
+"""Fused stroke-field rendering operator (no single counterpart in the reference).
+
+``render_strokes`` computes, for a batch of rays and a mixed-type stroke set,
+exactly what the reference computes with the chain
+``StrokeField.predict_density`` (``source/models.py:660-711``: one
+``stroke_fn`` call per stroke type, outputs placed by stroke index,
+``compose_strokes(..., 'over')``) followed by ``render.compute_alpha_weights``
+and ``render.volumetric_rendering`` (``source/utils/render.py:29-88``):
+per-ray ``rgb``, ``acc`` and ``depth``.  It runs as one gfx950 forward kernel
+and one adjoint kernel (plus a tiny stroke-table preparation and a gradient
+reduction); no per-(sample, stroke) tensor is materialised.
+
+Stroke sets are passed as ``stroke_ids`` (the reference's ``sdf_id`` per
+stroke, ``strokes.py:95-106``), ``shape_params`` padded to a common width,
+``color_params`` of one colour type and ``density_params`` (already scaled by
+``density_scale``).
+"""
+import torch
+from torch.autograd import Function
+
+from . import _lib
+from .strokelib.strokes import get_adjoint
+
+COLOR_WIDTH = [3, 12, 12, 27]
+N_BASE = 12
+
+
+def prepare_table(stroke_ids, shape_params):
+    """Per-stroke record table [S, 64] consumed by the render kernels."""
+    S, Pmax = shape_params.shape
+    table = torch.empty(S, 64, dtype=torch.float32, device=shape_params.device)
+    status = torch.zeros(1, dtype=torch.int32, device=shape_params.device)
+    _lib.check(_lib.lib().n3ds_render_prepare(_lib.ptr(table), _lib.ptr(stroke_ids), _lib.ptr(shape_params), S, Pmax,
+                                              _lib.ptr(status), _lib.stream()))
+    return table, status
+
+
+class _render_fn(Function):
+
+    @staticmethod
+    def forward(ctx, origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
+                density_params, cfg):
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters) = cfg
+        R, n1 = tdist.shape
+        S, Pmax = shape_params.shape
+        for name, t in (('origins', origins), ('directions', directions), ('viewdirs', viewdirs), ('radii', radii),
+                        ('tdist', tdist), ('shape_params', shape_params), ('color_params', color_params),
+                        ('density_params', density_params)):
+            _lib.require_cuda(name, t)
+        _lib.require_cuda('stroke_ids', stroke_ids, dtype=torch.int32)
+        dev = origins.device
+        table, _ = prepare_table(stroke_ids, shape_params)
+        bgt = (torch.tensor(bg, dtype=torch.float32) if not torch.is_tensor(bg) else bg.float().cpu())
+        bgv = (_lib.ctypes.c_float * 3)(*[float(v) for v in bgt.reshape(-1).expand(3)])
+        rgb = torch.empty(R, 3, dtype=torch.float32, device=dev)
+        acc = torch.empty(R, dtype=torch.float32, device=dev)
+        depth = torch.empty(R, dtype=torch.float32, device=dev)
+        state = torch.empty(5, R * (n1 - 1), dtype=torch.float32, device=dev)
+        _lib.check(_lib.lib().n3ds_render_forward(
+            _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
+            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), R, n1 - 1, S, Pmax,
+            color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
+            _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), float(cull_eps), _lib.ptr(rgb), _lib.ptr(acc),
+            _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.stream()))
+        ctx.save_for_backward(origins, directions, viewdirs, radii, tdist, table, shape_params, color_params,
+                              density_params, state)
+        ctx.cfg = cfg
+        ctx.bgv = bgv
+        return rgb, acc, depth
+
+    @staticmethod
+    def backward(ctx, g_rgb, g_acc, g_depth):
+        (origins, directions, viewdirs, radii, tdist, table, shape_params, color_params, density_params,
+         state) = ctx.saved_tensors
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters) = ctx.cfg
+        R, n1 = tdist.shape
+        S, Pmax = shape_params.shape
+        Q = color_params.shape[1]
+        dev = origins.device
+        g_rgb = torch.zeros(R, 3, device=dev) if g_rgb is None else g_rgb.float().contiguous()
+        g_acc = None if g_acc is None else g_acc.float().contiguous()
+        g_depth = None if g_depth is None else g_depth.float().contiguous()
+        nws = _lib.lib().n3ds_render_backward_workspace_floats(R, S, Pmax, Q)
+        if nws < 0:
+            _lib.check(2)
+        ws = torch.empty(max(int(nws), 1), dtype=torch.float32, device=dev)
+        gs = torch.zeros_like(shape_params)
+        gc = torch.zeros_like(color_params)
+        gd = torch.zeros_like(density_params)
+        _lib.check(_lib.lib().n3ds_render_backward(
+            _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
+            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), R, n1 - 1, S, Pmax, Q, color_id,
+            float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
+            _lib.ctypes.cast(ctx.bgv, _lib.ctypes.c_void_p), float(cull_eps), 1 if adjoint == 'exact' else 0,
+            _lib.ptr(g_rgb), _lib.ptr(g_acc), _lib.ptr(g_depth), _lib.ptr(state), _lib.ptr(ws), int(ws.numel()),
+            _lib.ptr(gs), _lib.ptr(gc), _lib.ptr(gd), _lib.stream()))
+        return (None, None, None, None, None, None, gs if ctx.needs_input_grad[6] else None,
+                gc if ctx.needs_input_grad[7] else None, gd if ctx.needs_input_grad[8] else None, None)
+
+
+def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
+                   density_params, color_id=0, sdf_delta=1.0, use_laplace_transform=False, inv_scale_radius=False,
+                   cone=False, coord_div=1.0, bg_color=(1.0, 1.0, 1.0), cull_eps=1e-7, adjoint=None,
+                   counters=None):
+    """Render rays through a stroke field; returns (rgb [R,3], acc [R], depth [R]).
+
+    origins/directions/viewdirs [R,3], radii [R] (sample radius = radii·t if ``cone``
+    else radii, as in ``render.cast_rays``), tdist [R, n+1] fenceposts,
+    stroke_ids int32 [S], shape_params [S, Pmax], color_params [S, Q],
+    density_params [S].  Sample coordinates are ``(o + d·t) / coord_div`` (the
+    ``_warp_coords`` bound of ``models.py:19-33``).  Gradients flow to
+    shape_params, color_params and density_params.
+    """
+    if color_params.shape[1] != COLOR_WIDTH[color_id]:
+        raise RuntimeError(f'color_params has width {color_params.shape[1]}, color_id {color_id} needs '
+                           f'{COLOR_WIDTH[color_id]}')
+    if stroke_ids.device.type == 'cpu':
+        ids = stroke_ids.tolist()
+        for i in ids:
+            base, flags = i >> 4, i & 15
+            if base >= N_BASE or (base >= 9 and flags) or ((flags & 4) and (flags & 8)):
+                raise RuntimeError(f'invalid stroke id {i}')
+        stroke_ids = stroke_ids.to(device=shape_params.device, dtype=torch.int32)
+    cfg = (int(color_id), float(sdf_delta), bool(use_laplace_transform), bool(inv_scale_radius), bool(cone),
+           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters)
+    f = lambda t: t.contiguous().float()
+    return _render_fn.apply(f(origins), f(directions), f(viewdirs), f(radii), f(tdist), stroke_ids.contiguous(),
+                            f(shape_params), f(color_params), f(density_params), cfg)

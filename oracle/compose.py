@@ -1,0 +1,169 @@
+"""Oracle restatement of stroke compositing and the ray march (test infrastructure only).
+
+Follows ``source/strokelib/src/compositing.cu:9-157`` (ordered "over"
+compositing and its hand-written adjoint), ``source/strokelib/strokes.py:388-417``
+(non-"over" modes) and ``source/utils/render.py:5-116`` (cast_rays,
+compute_alpha_weights, volumetric_rendering).
+
+Pinned by ``tests/test_oracle_compose.py``: a single stroke returns its own
+colour and ``D = d·α``; all-zero α gives ``D = 0`` and ``C = 0``; the sequential
+over-operator equals the closed form with a reversed exclusive ``cumprod``; the
+ray march equals ``1 - exp`` / ``cumsum`` closed forms and ``Σw ≤ 1``; the
+reference adjoint's density part equals the autograd derivative while its
+colour part differs (code-evident, only equal for one stroke).
+"""
+import torch
+
+# The reference evaluates ``1.0f + 1e-6f - T`` in float32, so the constant it
+# actually uses is float32(1 + 1e-6) = 1.00000095367431640625 (compositing.cu:49,106).
+_ONE_PLUS_EPS = 1.00000095367431640625
+
+
+def compose_forward(alphas, colors, density_params, skip_zero=True):
+    """compose_forward_kernel, compositing.cu:18-58.
+
+    alphas [N,S], colors [N,S,D], density_params [S].  Strokes are visited from
+    index S-1 down to 0 (higher index in front).  Returns (density [N],
+    color [N,D], raw color sum [N,D], final transmittance [N]).  ``skip_zero``
+    reproduces the kernel's skip of alpha == 0 (identical values; the exact
+    adjoint differentiates the unskipped form, whose derivative at alpha = 0 is
+    the one-sided limit).
+    """
+    N, S = alphas.shape
+    T = torch.ones(N, dtype=alphas.dtype)
+    dens = torch.zeros(N, dtype=alphas.dtype)
+    col = torch.zeros(N, colors.shape[-1], dtype=alphas.dtype)
+    for k in range(S - 1, -1, -1):
+        a = alphas[:, k]
+        nz = (a != 0.0) if skip_zero else torch.ones_like(a, dtype=torch.bool)
+        w = a * T
+        T = torch.where(nz, T * (1.0 - a), T)
+        dens = torch.where(nz, dens + density_params[k] * w, dens)
+        col = torch.where(nz[:, None], col + colors[:, k, :] * w[:, None], col)
+    scale = 1.0 / (_ONE_PLUS_EPS - T)
+    out = torch.clamp(col * scale[:, None], 0.0, 1.0)
+    return dens, out, col, T
+
+
+def compose_closed_form(alphas, colors, density_params):
+    """Same quantity via exclusive cumprod over the reversed stroke axis (a textbook reduction)."""
+    a = torch.flip(alphas, [-1])
+    c = torch.flip(colors, [-2])
+    d = torch.flip(density_params, [-1])
+    T_excl = torch.cumprod(torch.cat([torch.ones_like(a[:, :1]), 1.0 - a[:, :-1]], -1), -1)
+    w = a * T_excl
+    dens = (w * d).sum(-1)
+    col = (w[..., None] * c).sum(-2)
+    Tn = torch.prod(1.0 - a, -1)
+    out = torch.clamp(col / (_ONE_PLUS_EPS - Tn)[:, None], 0.0, 1.0)
+    return dens, out
+
+
+def compose_backward_reference(grad_density, grad_color, alphas, colors, density_params):
+    """Restatement of compose_backward_kernel, compositing.cu:73-156 (colour→α term as written)."""
+    N, S = alphas.shape
+    Dc = colors.shape[-1]
+    dens, _, col, Tn = compose_forward(alphas, colors, density_params)
+    if grad_density is None or grad_density.numel() == 0:
+        grad_density = torch.zeros(N, dtype=alphas.dtype)
+    if grad_color is None or grad_color.numel() == 0:
+        grad_color = torch.zeros(N, Dc, dtype=alphas.dtype)
+    O = _ONE_PLUS_EPS - Tn
+    scale = 1.0 / O
+    scaled = col * scale[:, None]
+    in_range = (scaled >= 0.0) & (scaled <= 1.0)
+    gC = torch.where(in_range, grad_color, torch.zeros_like(grad_color))
+    T = torch.ones(N, dtype=alphas.dtype)
+    d2 = torch.zeros(N, dtype=alphas.dtype)
+    c2 = torch.zeros(N, Dc, dtype=alphas.dtype)
+    g_alpha = torch.zeros_like(alphas)
+    g_colors = torch.zeros_like(colors)
+    g_dp = torch.zeros_like(density_params)
+    for k in range(S - 1, -1, -1):
+        a = alphas[:, k]
+        pos = a > 0.0
+        w = a * T
+        T = torch.where(pos, T * (1.0 - a), T)
+        d2 = torch.where(pos, d2 + density_params[k] * w, d2)
+        c2 = torch.where(pos[:, None], c2 + colors[:, k, :] * w[:, None], c2)
+        g_dp[k] = torch.where(pos, grad_density * w, torch.zeros_like(w)).sum()
+        g_colors[:, k, :] = torch.where(pos[:, None], gC * scale[:, None] * w[:, None], torch.zeros_like(gC))
+        sc = 1.0 / torch.clamp_min(1.0 - a, 1e-6)
+        ga = grad_density * (T * density_params[k] - (dens - d2)) * sc
+        sc2 = sc * scale * scale
+        for i in range(Dc):
+            ga = ga + gC[:, i] * (T * colors[:, k, i] * (O - a) - (col[:, i] - c2[:, i])) * sc2
+        g_alpha[:, k] = ga
+    return g_alpha, g_colors, g_dp
+
+
+def compose_backward_exact(grad_density, grad_color, alphas, colors, density_params):
+    """True derivative of :func:`compose_forward` by torch.autograd (use float64)."""
+    with torch.enable_grad():
+        a = alphas.detach().clone().requires_grad_(True)
+        c = colors.detach().clone().requires_grad_(True)
+        d = density_params.detach().clone().requires_grad_(True)
+        dens, out, _, _ = compose_forward(a, c, d, skip_zero=False)
+        loss = 0.0
+        if grad_density is not None and grad_density.numel() > 0:
+            loss = loss + (dens * grad_density).sum()
+        if grad_color is not None and grad_color.numel() > 0:
+            loss = loss + (out * grad_color).sum()
+        g = torch.autograd.grad(loss, [a, c, d], allow_unused=True)
+    return tuple(gi if gi is not None else torch.zeros_like(x) for gi, x in zip(g, [a, c, d]))
+
+
+def compose_other(alphas, colors, density_params, composition_type):
+    """Non-"over" composition modes, strokes.py:390-417."""
+    if composition_type == 'max':
+        idx = torch.argmax(alphas, dim=-1, keepdim=True)
+        a = torch.take_along_dim(alphas, idx, dim=-1).squeeze(-1)
+        return a * density_params[idx].squeeze(-1), torch.take_along_dim(colors, idx[..., None], dim=-2).squeeze(-2)
+    if composition_type == 'max_density_weighted':
+        idx = torch.argmax(alphas * density_params, dim=-1, keepdim=True)
+        a = torch.take_along_dim(alphas, idx, dim=-1).squeeze(-1)
+        return a * density_params[idx].squeeze(-1), torch.take_along_dim(colors, idx[..., None], dim=-2).squeeze(-2)
+    if composition_type in ('softmax', 'softmax_density_weighted'):
+        logits = alphas if composition_type == 'softmax' else alphas * density_params
+        w = torch.softmax(logits * (1.0 / 0.05), dim=-1)
+        return (w * alphas * density_params).sum(-1), (w[..., None] * colors).sum(-2)
+    raise ValueError(composition_type)
+
+
+# ----------------------------------------------------------------------------
+# ray march, render.py:5-116
+# ----------------------------------------------------------------------------
+
+def cast_rays(tdist, origins, directions, radii):
+    """render.py:5-26."""
+    t0 = tdist[..., :-1]
+    t1 = tdist[..., 1:]
+    t = (t0 + t1) / 2
+    coords = directions[..., None, :] * t[..., None] + origins[..., None, :]
+    radius = radii * t
+    return coords, radius, t
+
+
+def compute_alpha_weights(density, tdist, dirs, opaque_background=False):
+    """render.py:29-48."""
+    t_delta = tdist[..., 1:] - tdist[..., :-1]
+    delta = t_delta * torch.norm(dirs[..., None, :], dim=-1)
+    density_delta = density * delta
+    if opaque_background:
+        density_delta = torch.cat([density_delta[..., :-1], torch.full_like(density_delta[..., -1:], torch.inf)], -1)
+    alpha = 1 - torch.exp(-density_delta)
+    trans = torch.exp(-torch.cat([torch.zeros_like(density_delta[..., :1]),
+                                  torch.cumsum(density_delta[..., :-1], dim=-1)], dim=-1))
+    return alpha * trans, alpha, trans
+
+
+def volumetric_rendering(rgbs, weights, tdist, bg_rgbs):
+    """render.py:73-88 (rgb, depth, acc)."""
+    eps = torch.finfo(rgbs.dtype).eps
+    acc = weights.sum(dim=-1)
+    bg_w = (1 - acc[..., None]).clamp_min(0.)
+    rgb = (weights[..., None] * rgbs).sum(dim=-2) + bg_w * bg_rgbs
+    t_mids = 0.5 * (tdist[..., :-1] + tdist[..., 1:])
+    depth = torch.clip(torch.nan_to_num((weights * t_mids).sum(dim=-1) / acc.clamp_min(eps), torch.inf),
+                       tdist[..., 0], tdist[..., -1])
+    return rgb, acc, depth

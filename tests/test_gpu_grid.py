@@ -1,0 +1,94 @@
+"""HIP hash-grid encoder (drop-in ``GridEncoder`` / ``grid_encode``) against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_err
+from neural3dstrokes_amd.gridencoder import GridEncoder
+from oracle import grid as ogrid
+
+
+def _enc(dev, **kw):
+    torch.manual_seed(0)
+    enc = GridEncoder(**kw).to(dev)
+    return enc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('kw', [dict(), dict(desired_resolution=2048, log2_hashmap_size=19),
+                                dict(num_levels=8, level_dim=4, log2_hashmap_size=15, gridtype='tiled'),
+                                dict(num_levels=6, level_dim=2, interpolation='smoothstep', align_corners=True),
+                                dict(input_dim=2, num_levels=6, level_dim=1, log2_hashmap_size=12)])
+def test_grid_forward_backward(kw, dev):
+    enc = _enc(dev, **kw)
+    g = torch.Generator().manual_seed(1)
+    B = 4096
+    D = enc.input_dim
+    x = torch.rand(B, D, generator=g) * 2.2 - 1.1  # includes out-of-range points (zero output)
+    out = enc(x.to(dev))
+    gout = torch.randn(out.shape, generator=g)
+    out.backward(gout.to(dev))
+    inp = ((x + 1) / 2).numpy()
+    ref, _ = ogrid.grid_forward(inp, enc.embeddings.detach().cpu().numpy(), enc.offsets.cpu().numpy(),
+                                enc.per_level_scale, enc.base_resolution, enc.gridtype_id, enc.align_corners,
+                                enc.interp_id, f32=True)
+    assert rel_err(out, torch.from_numpy(ref)) < 1e-5
+    gref, _ = ogrid.grid_backward(gout.numpy(), inp, enc.embeddings.detach().cpu().numpy(), enc.offsets.cpu().numpy(),
+                                  enc.per_level_scale, enc.base_resolution, enc.gridtype_id, enc.align_corners,
+                                  enc.interp_id, f32=True)
+    assert rel_err(enc.embeddings.grad, torch.from_numpy(gref)) < 1e-5
+
+
+@pytest.mark.gpu
+def test_grid_fp16_tables(dev):
+    """fp16 storage, fp32 interpolation / accumulation; error against float64 stated per output."""
+    enc = _enc(dev, desired_resolution=2048)
+    g = torch.Generator().manual_seed(2)
+    x = torch.rand(8192, 3, generator=g)
+    emb16 = enc.embeddings.detach().half().requires_grad_(True)
+    from neural3dstrokes_amd.gridencoder import grid_encode
+    out = grid_encode(x.to(dev), emb16, enc.offsets, enc.per_level_scale, enc.base_resolution, False, 0, False, 0)
+    assert out.dtype == torch.float16
+    gout = torch.randn(out.shape, generator=g)
+    out.backward(gout.to(dev).half())
+    ref, _ = ogrid.grid_forward(x.numpy(), emb16.detach().float().cpu().numpy(), enc.offsets.cpu().numpy(),
+                                enc.per_level_scale, enc.base_resolution)
+    # fp16 output rounding: relative error bounded by half-precision epsilon
+    assert rel_err(out.float(), torch.from_numpy(ref)) < 1e-3
+    gref, _ = ogrid.grid_backward(gout.half().float().numpy(), x.numpy(), emb16.detach().float().cpu().numpy(),
+                                  enc.offsets.cpu().numpy(), enc.per_level_scale, enc.base_resolution)
+    assert emb16.grad.dtype == torch.float16
+    assert rel_err(emb16.grad.float(), torch.from_numpy(gref)) < 2e-3
+
+
+@pytest.mark.gpu
+def test_grid_input_grad(dev):
+    enc = _enc(dev, num_levels=4, level_dim=2, log2_hashmap_size=14)
+    enc.embeddings.data.uniform_(-1, 1)
+    g = torch.Generator().manual_seed(3)
+    x = (torch.rand(512, 3, generator=g) * 1.8 - 0.9).to(dev).requires_grad_(True)
+    out = enc(x)
+    gout = torch.randn(out.shape, generator=g).to(dev)
+    out.backward(gout)
+    inp = ((x.detach().cpu() + 1) / 2).numpy()
+    _, dydx = ogrid.grid_forward(inp, enc.embeddings.detach().cpu().numpy(), enc.offsets.cpu().numpy(),
+                                 enc.per_level_scale, enc.base_resolution, calc_dy_dx=True, f32=True)
+    _, gin = ogrid.grid_backward(gout.cpu().numpy(), inp, enc.embeddings.detach().cpu().numpy(),
+                                 enc.offsets.cpu().numpy(), enc.per_level_scale, enc.base_resolution, dy_dx=dydx,
+                                 f32=True)
+    # grid.py maps inputs with (x + 1) / 2, so dL/dx = 0.5 * dL/dinput
+    assert rel_err(x.grad, torch.from_numpy(gin) * 0.5) < 1e-4
+
+
+@pytest.mark.gpu
+def test_grid_total_variation(dev):
+    enc = _enc(dev, num_levels=4, level_dim=2, log2_hashmap_size=12)
+    enc.embeddings.data.uniform_(-1, 1)
+    enc.embeddings.grad = torch.zeros_like(enc.embeddings)
+    g = torch.Generator().manual_seed(4)
+    x = torch.rand(2048, 3, generator=g) * 2 - 1
+    enc.grad_total_variation(weight=1e-3, inputs=x.to(dev))
+    ref = ogrid.grad_total_variation(((x + 1) / 2).numpy(), enc.embeddings.detach().cpu().numpy(),
+                                     np.zeros(tuple(enc.embeddings.shape)), enc.offsets.cpu().numpy(), 1e-3,
+                                     enc.per_level_scale, enc.base_resolution)
+    assert rel_err(enc.embeddings.grad, torch.from_numpy(ref)) < 1e-4

@@ -66,7 +66,9 @@ def grid_index(gridtype, align_corners, hashmap_size, resolution, pos_grid):
         if stride > hashmap_size:
             break
         index = (index + pos_grid[..., d].astype(np.uint64) * np.uint64(stride)) & np.uint64(0xFFFFFFFF)
-        stride *= resolution if align_corners else resolution + 1
+        # uint32 product, as in the CUDA code: for resolution+1 = 65537 the stride
+        # wraps to 131073 <= hashmap_size and the level keeps a (wrapped) dense index
+        stride = (stride * (resolution if align_corners else resolution + 1)) & 0xFFFFFFFF
     if gridtype == 0 and stride > hashmap_size:
         index = fast_hash(pos_grid).astype(np.uint64)
     return (index % np.uint64(hashmap_size)).astype(np.int64)

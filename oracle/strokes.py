@@ -426,6 +426,57 @@ def _polyline_t(p, point_fn, A, B, C, D, like):
     return t_min
 
 
+def curve_selection_margin(base, p, prm):
+    """Relative gap between the closest and the runner-up curve-parameter candidate.
+
+    Cubic / Catmull-Rom: distances to the ten polyline segments (candidates with
+    a different parameter t); quadratic: the two real roots of the three-root
+    branch (inf in the one-root branch).  A gap below float32 resolution means
+    any float32 evaluation may select either candidate.
+    """
+    if base == 9:
+        a, b, c, d, kx, ky, kz, pp, q, h = _quad_setup(p, prm)
+        one = h >= 0.0
+        pn = torch.where(one, -torch.ones_like(pp), pp)
+        z = torch.sqrt(-pn + 1e-8)
+        arg = torch.clamp(torch.where(one, torch.zeros_like(q), q) / (pn * z * 2.0 + 1e-8), -1.0, 1.0)
+        v = torch.acos(arg) / 3.0
+        m, n = torch.cos(v), torch.sin(v) * 1.732050808
+        ta = _clamp((m + n) * z - kx, 0.0, 1.0)
+        tb = _clamp((-n - m) * z - kx, 0.0, 1.0)
+        da, db = _quad_dist2(d, c, b, ta), _quad_dist2(d, c, b, tb)
+        gap = (da - db).abs() / torch.maximum(da, db).clamp_min(1e-30)
+        gap = torch.where((ta - tb).abs() > 1e-9, gap, torch.full_like(gap, float('inf')))
+        return torch.where(one, torch.full_like(gap, float('inf')), gap)
+    if base == 10:
+        A, B, C, D = _cubic_ctrl(prm)
+        fn = _cubic_point
+    else:
+        A, B, C, D = catmull_coeffs(prm)
+        fn = _catmull_point
+    like = p[0] + prm[..., 0] * 0
+    step = 1.0 / CURVE_SEGMENTS
+    dists, ts = [], []
+    s0 = fn(torch.zeros_like(like), A, B, C, D)
+    for i in range(1, CURVE_SEGMENTS + 1):
+        s1 = fn(torch.zeros_like(like) + i * step, A, B, C, D)
+        ab = [s1[k] - s0[k] for k in range(3)]
+        pa = [p[k] - s0[k] for k in range(3)]
+        ts_ = _clamp(_dot(pa, ab) / _dot(ab, ab), 0.0, 1.0)
+        diff = [p[k] - (s0[k] + ts_ * ab[k]) for k in range(3)]
+        dists.append(_safe_sqrt(_dot(diff, diff)))
+        ts.append((i - 1) * step * (1.0 - ts_) + i * step * ts_)
+        s0 = s1
+    dd = torch.stack(dists, -1)
+    tt = torch.stack(ts, -1)
+    best = dd.argmin(-1, keepdim=True)
+    d1 = torch.gather(dd, -1, best)
+    t1 = torch.gather(tt, -1, best)
+    other = torch.where((tt - t1).abs() > 1e-9, dd, torch.full_like(dd, float('inf')))
+    d2 = other.min(-1, keepdim=True).values
+    return ((d2 - d1) / d1.clamp_min(1e-30)).squeeze(-1)
+
+
 def _cubic_ctrl(prm):
     return ([prm[..., 0], prm[..., 1], prm[..., 2]], [prm[..., 3], prm[..., 4], prm[..., 5]],
             [prm[..., 6], prm[..., 7], prm[..., 8]], [prm[..., 9], prm[..., 10], prm[..., 11]])

@@ -72,7 +72,7 @@ def _grads(shape, color, adjoint, laplace, isr, dev, seed=3, delta=1.0):
     gc = rand_like_seeded((x.shape[0], x.shape[1], S, 3), seed + 11)
     gs = rand_like_seeded((x.shape[0], x.shape[1], S), seed + 12)
     ok = conditioning_mask(shape, sp, x, r, vd, laplace, delta).reshape(ga.shape)
-    assert (~ok).sum() <= 0.01 * ok.numel(), (~ok).sum()
+    assert (~ok).sum() <= 0.05 * ok.numel(), (~ok).sum()  # excluded pairs: see helpers.conditioning_mask
     ga = ga * ok
     gs = gs * ok
     dstrokes.set_adjoint(adjoint)

@@ -1,0 +1,258 @@
+#!/usr/bin/env python
+"""Stroke-field hot-path benchmark: fused HIP fwd+bwd vs the plain per-op PyTorch formulation.
+
+Workload (default): 500 strokes (200 cubic Bezier tubes, 100 quadratic Bezier
+tubes, 200 solids from {ellipsoid, box, round box, octahedron, tetrahedron}
+with per-axis scale), 65536 rays x 128 stratified samples, linear alpha with
+sdf_delta * radius = 0.01, constant RGB, N(0,1) upstream grads on rgb / acc.
+A "step" is one forward + backward of the fused renderer (parameter
+gradients allocated and zeroed inside the step).
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(HIP events around each launch on the launching stream), the plain per-op
+PyTorch-ROCm comparison on the same GPU, the fused-vs-plain gradient error
+and a CPU baseline (the float32 CPU oracle on a bounded ray sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = 'fwd+bwd ray-samples/s at 500 strokes, fused vs plain, grad rel err ≤1e-4'
+# frozen flop model per useful (sample, stroke) pair with alpha > 0: (forward, forward+backward);
+# convention add/mul/min/max/select = 1, FMA = 2, each transcendental = 1
+FLOPS = {0: (44, 204), 1: (55, 230), 2: (56, 233), 4: (47, 212), 7: (43, 198), 8: (45, 203),
+         9: (115, 445), 10: (334, 824), 11: (334, 824), 3: (60, 240), 5: (50, 215), 6: (45, 200)}
+MARCH_FWD, MARCH_BWD = 10, 15  # per ray-sample volume-render flops (25 fwd+bwd)
+PEAK_FP32 = 157.3e12
+BASE_NAMES = ['sphere', 'cube', 'round_cube', 'capped_torus', 'capsule', 'line', 'triprism', 'octahedron',
+              'tetrahedron', 'quadratic_bezier', 'cubic_bezier', 'catmull_rom']
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return ((a - b).norm() / b.norm().clamp_min(1e-30)).item()
+
+
+def accuracy_block(args, sc, T, ids, gr, ga, n):
+    """Fused and plain float32 against a float64 reference on the first ``acc_rays`` rays.
+
+    The float64 reference is the plain per-op formulation evaluated in float64
+    on the GPU (the gradient check).  A stroke counts as ill-conditioned when
+    its float64 parameter gradient moves by more than 1e-4 (relative) under a
+    1e-7 relative perturbation of all stroke parameters, i.e. when float32
+    rounding alone can move it past the tolerance; errors are reported with and
+    without those strokes.
+    """
+    import torch
+    from neural3dstrokes_amd import plain, render
+    ar = min(args.acc_rays, T['tdist'].shape[0])
+    sl = slice(0, ar)
+    types = torch.tensor(sc['types'])
+
+    def run_plain(dt, sp):
+        return plain.fwd_bwd(T['origins'][sl].to(dt), T['directions'][sl].to(dt), T['radii'][sl].to(dt),
+                             T['tdist'][sl].to(dt), sc['stroke_ids'], sp.to(dt), T['color_params'].to(dt),
+                             T['density_params'].to(dt), gr[sl].to(dt), ga[sl].to(dt), chunk=256)
+
+    ref = run_plain(torch.float64, T['shape_params'])
+    gen = torch.Generator().manual_seed(7)
+    noise = 1 + 1e-7 * torch.randn(T['shape_params'].shape, generator=gen, dtype=torch.float64)
+    pert = run_plain(torch.float64, T['shape_params'].double() * noise.to(T['shape_params'].device))
+    sens = ((pert[2] - ref[2]).norm(dim=-1) / ref[2].norm(dim=-1).clamp_min(1e-30)).cpu()
+    ok = sens <= 1e-4
+    p32 = run_plain(torch.float32, T['shape_params'])
+    sp = T['shape_params'].clone().requires_grad_(True)
+    cp = T['color_params'].clone().requires_grad_(True)
+    dp = T['density_params'].clone().requires_grad_(True)
+    rgb, acc, _ = render.render_strokes(T['origins'][sl], T['directions'][sl], T['viewdirs'][sl], T['radii'][sl],
+                                        T['tdist'][sl], ids, sp, cp, dp, adjoint='exact')
+    torch.autograd.backward([rgb, acc], [gr[sl], ga[sl]])
+    fused = (rgb, acc, sp.grad, cp.grad, dp.grad)
+    names = ('rgb', 'acc', 'grad_shape', 'grad_color', 'grad_density')
+    okd = ok.to(sp.device)
+
+    def errs(x):
+        e = {nm: rel(x[i], ref[i]) for i, nm in enumerate(names)}
+        e['grad_shape_conditioned'] = rel(x[2][okd], ref[2][okd])
+        e['grad_shape_by_type'] = {BASE_NAMES[b]: rel(x[2][(types == b).to(sp.device)], ref[2][(types == b).to(sp.device)])
+                                   for b in sorted(set(sc['types']))}
+        return e
+
+    f_vs_p = {nm: rel(fused[i], p32[i]) for i, nm in enumerate(names)}
+    f_vs_p['grad_shape_conditioned'] = rel(fused[2][okd], p32[2][okd])
+    return {'rays': ar, 'reference': 'plain formulation in float64 on the GPU',
+            'ill_conditioned_strokes': int((~ok).sum()),
+            'ill_conditioned_types': {BASE_NAMES[b]: int(((types == b) & ~ok).sum()) for b in sorted(set(sc['types']))},
+            'fused_vs_fp64': errs(fused), 'plain_fp32_vs_fp64': errs(p32), 'fused_vs_plain_fp32': f_vs_p}
+
+
+def cpu_baseline(sc, n_rays, g_rgb, g_acc):
+    """Float32 CPU oracle (plain per-op math, no culling) fwd+bwd on the first n_rays rays."""
+    import torch
+    from oracle import field as ofield
+    threads = min(os.cpu_count() or 1, 16)
+    torch.set_num_threads(threads)
+    d = torch.float32
+    args = [sc[k][:n_rays].to(d) for k in ('origins', 'directions', 'viewdirs', 'radii', 'tdist')]
+    call = lambda: ofield.render_field_grads(g_rgb[:n_rays].cpu(), g_acc[:n_rays].cpu(), *args, sc['stroke_ids'],
+                                             sc['shape_params'].to(d), sc['color_params'].to(d),
+                                             sc['density_params'].to(d))
+    call()
+    t0 = time.perf_counter()
+    call()
+    dt = time.perf_counter() - t0
+    n = n_rays * (sc['tdist'].shape[1] - 1)
+    return dict(value=n / dt, unit='ray-samples/s', cores=threads, kind='port',
+                sample=f'{n_rays} rays x {sc["tdist"].shape[1] - 1} samples x {len(sc["stroke_ids"])} strokes '
+                       f'(first rays of the workload), float32 CPU oracle fwd+bwd (autograd), 1 warmup + 1 timed',
+                seconds=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='cfg1')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--adjoint', default='exact', choices=['exact', 'reference'])
+    ap.add_argument('--plain-rays', type=int, default=8192, help='rays for the plain per-op comparator (0: skip)')
+    ap.add_argument('--plain-chunk', type=int, default=1024)
+    ap.add_argument('--cpu-rays', type=int, default=256, help='rays for the CPU baseline (0: skip)')
+    ap.add_argument('--fused-only', action='store_true', help='only the fused steps (for profiler runs)')
+    ap.add_argument('--acc-rays', type=int, default=4096, help='rays for the float64 accuracy check (0: skip)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build measures one GPU; multi-GPU runs are not part of it',
+                          'n_gpus': args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from neural3dstrokes_amd import render, scenes
+
+    dev = torch.device('cuda:0')
+    sc = scenes.make_workload(args.workload, seed=args.seed)
+    R, n1 = sc['tdist'].shape
+    n = n1 - 1
+    S = len(sc['stroke_ids'])
+    g = torch.Generator().manual_seed(args.seed + 1)
+    g_rgb = torch.randn(R, 3, generator=g)
+    g_acc = torch.randn(R, generator=g)
+    T = {k: sc[k].to(dev) for k in ('origins', 'directions', 'viewdirs', 'radii', 'tdist', 'shape_params',
+                                     'color_params', 'density_params')}
+    ids = sc['stroke_ids'].to(dev)
+    gr, ga = g_rgb.to(dev), g_acc.to(dev)
+    sp = T['shape_params'].clone().requires_grad_(True)
+    cp = T['color_params'].clone().requires_grad_(True)
+    dp = T['density_params'].clone().requires_grad_(True)
+
+    def fused(counters=None, ev=None):
+        sp.grad = cp.grad = dp.grad = None
+        if ev:
+            ev[0].record()
+        rgb, acc, depth = render.render_strokes(T['origins'], T['directions'], T['viewdirs'], T['radii'], T['tdist'],
+                                                ids, sp, cp, dp, adjoint=args.adjoint, counters=counters)
+        if ev:
+            ev[1].record()
+        torch.autograd.backward([rgb, acc], [gr, ga])
+        if ev:
+            ev[2].record()
+        return rgb, acc
+
+    for _ in range(args.warmup):
+        fused()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0.record()
+    for i in range(args.steps):
+        fused(ev=evs[i])
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1) / args.steps
+    fwd_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    bwd_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    value = R * n / (ms * 1e-3)
+    if args.fused_only:
+        print(json.dumps({'metric': METRIC, 'value': value, 'ms_per_step': ms, 'fwd_ms': fwd_ms, 'bwd_ms': bwd_ms}))
+        return
+
+    # useful-pair counts (alpha > 0) per stroke type, from an untimed counting pass
+    counters = torch.zeros(2 * 12 + 1, dtype=torch.int64, device=dev)
+    fused(counters=counters)
+    torch.cuda.synchronize()
+    cnt = counters.cpu().tolist()
+    pos_pairs = {b: cnt[b] for b in range(12) if cnt[b]}
+    eval_lanes = sum(cnt[12:24])
+    fwd_flops = sum(FLOPS[b][0] * c for b, c in pos_pairs.items()) + MARCH_FWD * R * n
+    bwd_flops = sum((FLOPS[b][1] - FLOPS[b][0]) * c for b, c in pos_pairs.items()) + MARCH_BWD * R * n
+    kernels = {'render_fwd': (fwd_flops, fwd_ms), 'render_bwd': (bwd_flops, bwd_ms)}
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    achieved = kernels[dom][0] / (kernels[dom][1] * 1e-3)
+    useful_total = fwd_flops + bwd_flops
+
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'ray-samples/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'fp32', 'data': 'synthetic (seeded stroke scene and cameras)',
+        'config': {'workload': f'{args.workload}: {S} strokes (mixed palette) x {R} rays x {n} samples, fused '
+                               f'fwd+bwd, {args.adjoint} adjoint', 'strokes': S, 'rays': R, 'samples_per_ray': n,
+                   'adjoint': args.adjoint},
+        'roofline': {'bound': 'valu', 'kernel': dom, 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': None,
+                     'flops_per_launch': kernels[dom][0], 'ms_per_launch': kernels[dom][1]},
+        'kernels_ms': {'render_fwd': fwd_ms, 'render_bwd': bwd_ms},
+        'useful_pairs': {BASE_NAMES[b]: c for b, c in pos_pairs.items()},
+        'evaluated_pair_lanes': eval_lanes,
+        'useful_gflop_per_step': useful_total / 1e9,
+        'wall_s_timed': wall,
+    }
+
+    if args.plain_rays > 0:
+        from neural3dstrokes_amd import plain
+        pr = min(args.plain_rays, R)
+        sl = slice(0, pr)
+        pargs = (T['origins'][sl], T['directions'][sl], T['radii'][sl], T['tdist'][sl], sc['stroke_ids'],
+                 T['shape_params'], T['color_params'], T['density_params'], gr[sl], ga[sl])
+        plain.fwd_bwd(*pargs, chunk=args.plain_chunk)  # warmup
+        torch.cuda.synchronize()
+        p0 = time.perf_counter()
+        prgb, pacc, pgs, pgc, pgd = plain.fwd_bwd(*pargs, chunk=args.plain_chunk)
+        torch.cuda.synchronize()
+        pdt = time.perf_counter() - p0
+        plain_value = pr * n / pdt
+        # fused on the same rays for the accuracy comparison
+        spf = T['shape_params'].clone().requires_grad_(True)
+        cpf = T['color_params'].clone().requires_grad_(True)
+        dpf = T['density_params'].clone().requires_grad_(True)
+        frgb, facc, _ = render.render_strokes(T['origins'][sl], T['directions'][sl], T['viewdirs'][sl],
+                                              T['radii'][sl], T['tdist'][sl], ids, spf, cpf, dpf,
+                                              adjoint=args.adjoint)
+        torch.autograd.backward([frgb, facc], [gr[sl], ga[sl]])
+        out['plain'] = {'value': plain_value, 'unit': 'ray-samples/s', 'rays': pr, 'chunk_rays': args.plain_chunk,
+                        's_per_call': pdt,
+                        'speedup_fused_vs_plain': value / plain_value,
+                        'rel_err_fused_vs_plain': {'rgb': rel(frgb, prgb), 'acc': rel(facc, pacc),
+                                                   'grad_shape': rel(spf.grad, pgs), 'grad_color': rel(cpf.grad, pgc),
+                                                   'grad_density': rel(dpf.grad, pgd)}}
+        if args.acc_rays > 0:
+            out['accuracy'] = accuracy_block(args, sc, T, ids, gr, ga, n)
+    if args.cpu_rays > 0:
+        out['cpu_baseline'] = cpu_baseline(sc, min(args.cpu_rays, R), g_rgb, g_acc)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

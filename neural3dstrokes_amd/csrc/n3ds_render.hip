@@ -768,6 +768,25 @@ __global__ void render_reduce_kernel(const float* __restrict__ partials, int npa
 static int g_num_cu = 0;
 static int g_max_lds = 0;
 
+typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
+
+static BwdKernel bwd_kernel_ptr(int ns, int cid) {
+    static const BwdKernel tab[4][4] = {
+        {render_bwd_kernel<1, 0>, render_bwd_kernel<1, 1>, render_bwd_kernel<1, 2>, render_bwd_kernel<1, 3>},
+        {render_bwd_kernel<2, 0>, render_bwd_kernel<2, 1>, render_bwd_kernel<2, 2>, render_bwd_kernel<2, 3>},
+        {render_bwd_kernel<3, 0>, render_bwd_kernel<3, 1>, render_bwd_kernel<3, 2>, render_bwd_kernel<3, 3>},
+        {render_bwd_kernel<4, 0>, render_bwd_kernel<4, 1>, render_bwd_kernel<4, 2>, render_bwd_kernel<4, 3>}};
+    return tab[ns - 1][cid];
+}
+
+// upper bound on persistent backward blocks: 8 per CU, never more than one per 4 rays
+static int64_t max_persistent_blocks(int64_t n_rays) {
+    int64_t nb = (int64_t)g_num_cu * 8;
+    int64_t need = (n_rays + RWAVES - 1) / RWAVES;
+    if (nb > need) nb = need;
+    return nb < 1 ? 1 : nb;
+}
+
 static int device_info() {
     if (g_num_cu == 0) {
         int dev;
@@ -856,11 +875,7 @@ N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_
     int64_t PG = shape_width + color_width + 1;
     int64_t lds = (int64_t)n_strokes * PG * 4;
     if (lds > g_max_lds) return (int64_t)n_strokes * PG;  // global-atomic path
-    int64_t nb = (int64_t)g_num_cu * 2;
-    int64_t need = (n_rays + RWAVES - 1) / RWAVES;
-    if (nb > need) nb = need;
-    if (nb < 1) nb = 1;
-    return nb * n_strokes * PG;
+    return max_persistent_blocks(n_rays) * n_strokes * PG;
 }
 
 N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directions, const float* viewdirs,
@@ -884,10 +899,20 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
     const int64_t PG = shape_width + color_width + 1;
     const int64_t lds = (int64_t)n_strokes * PG * 4;
     const int use_lds = lds <= g_max_lds;
-    int64_t nb = (int64_t)g_num_cu * 2;
-    int64_t need = (n_rays + RWAVES - 1) / RWAVES;
-    if (nb > need) nb = need;
-    if (nb < 1) nb = 1;
+    const int ns_ = (n_samples + 63) / 64;
+    int64_t nb = max_persistent_blocks(n_rays);
+    {
+        // as many persistent blocks as fit on the device at once (registers and LDS)
+        int per_cu = 0;
+        BwdKernel kq = bwd_kernel_ptr(ns_, color_id);
+        if (use_lds && lds > 65536)
+            N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kq, 64 * RWAVES,
+                                                                    use_lds ? (size_t)lds : 0));
+        if (per_cu < 1) per_cu = 1;
+        int64_t fit = (int64_t)g_num_cu * per_cu;
+        if (fit < nb) nb = fit;
+    }
     const int64_t ws_need = use_lds ? nb * n_strokes * PG : (int64_t)n_strokes * PG;
     N3DS_REQUIRE(workspace_floats >= ws_need, "workspace has %lld floats, need %lld", (long long)workspace_floats,
                  (long long)ws_need);
@@ -898,16 +923,10 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
     float* parts = use_lds ? workspace : nullptr;
     float* direct = use_lds ? nullptr : workspace;
     int ns = (n_samples + 63) / 64;
-    typedef void (*Bwd)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
-    static const Bwd tab[4][4] = {
-        {render_bwd_kernel<1, 0>, render_bwd_kernel<1, 1>, render_bwd_kernel<1, 2>, render_bwd_kernel<1, 3>},
-        {render_bwd_kernel<2, 0>, render_bwd_kernel<2, 1>, render_bwd_kernel<2, 2>, render_bwd_kernel<2, 3>},
-        {render_bwd_kernel<3, 0>, render_bwd_kernel<3, 1>, render_bwd_kernel<3, 2>, render_bwd_kernel<3, 3>},
-        {render_bwd_kernel<4, 0>, render_bwd_kernel<4, 1>, render_bwd_kernel<4, 2>, render_bwd_kernel<4, 3>}};
-    if (shm > 65536) N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)tab[ns - 1][color_id],
+    BwdKernel kern = bwd_kernel_ptr(ns, color_id);
+    if (shm > 65536) N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern,
                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(tab[ns - 1][color_id], grid, block, shm, s, A, grad_rgb, grad_acc, grad_depth, state, parts,
-                       use_lds, direct);
+    hipLaunchKernelGGL(kern, grid, block, shm, s, A, grad_rgb, grad_acc, grad_depth, state, parts, use_lds, direct);
     N3DS_LAUNCH_CHECK();
     int64_t tot = (int64_t)n_strokes * PG;
     hipLaunchKernelGGL(render_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace,

@@ -1,0 +1,187 @@
+/*
+ * n3ds.h - C-ABI of libn3ds.so, the gfx950 (MI355X) implementation of the
+ * Neural 3D Strokes stroke-field hot path.
+ *
+ * Conventions (all entry points):
+ *   - every pointer is a device pointer (hipMalloc / torch CUDA tensor storage)
+ *     unless stated otherwise; tensors are dense, row-major, float32 unless a
+ *     dtype argument says otherwise;
+ *   - an absent optional tensor is NULL (the reference passes an empty tensor);
+ *   - the caller allocates every output; outputs documented as "accumulated"
+ *     must be zero-initialised by the caller (the reference's torch.zeros);
+ *   - work is enqueued asynchronously on `stream` (a hipStream_t; NULL = the
+ *     legacy default stream); no entry point synchronises or allocates except
+ *     where noted;
+ *   - the return value is 0 on success, 1 for an invalid argument and 2 for a
+ *     HIP runtime error; n3ds_last_error() then returns the message (the
+ *     reference raises the TORCH_CHECK message as a Python RuntimeError).
+ *
+ * Reference files are cited relative to the reference repository root.
+ */
+#ifndef N3DS_H
+#define N3DS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message of the calling thread ("" if none). */
+const char* n3ds_last_error(void);
+
+/* ABI version of this library (1). */
+int n3ds_version(void);
+
+/* ------------------------------------------------------------------------
+ * Per-(point, stroke) stroke field.
+ * Replaces the pybind `stroke_forward` of
+ * source/strokelib/src/strokes.h:14-27 (kernel strokes_forward.cu:11-109,
+ * host strokes_forward.cu:170-225), called from strokes.py:146-183.
+ *
+ * alpha_out [N,S], color_out [N,S,3], sdf_out [N,S] or NULL,
+ * texcoord_out [N,S,2] or NULL; x [N,3], radius [N], viewdir [R,3] with
+ * N = R * samples (viewdir row = point / (N/R)); shape_params [S,P],
+ * color_params [S,Q].  sdf_id = base<<4 | T | R<<1 | SS<<2 | MS<<3,
+ * color_id 0..3 (constant_rgb, gradient_rgb, constant_sh2, constant_sh3).
+ * ------------------------------------------------------------------------ */
+int n3ds_stroke_forward(float* alpha_out, float* color_out, float* sdf_out, float* texcoord_out,
+                        const float* x, const float* radius, const float* viewdir,
+                        const float* shape_params, const float* color_params, int64_t n_points,
+                        int64_t n_viewdirs, int64_t n_strokes, int64_t n_shape_params,
+                        int64_t n_color_params, uint32_t sdf_id, uint32_t color_id, float sdf_delta,
+                        int use_laplace_transform, int inv_scale_radius, void* stream);
+
+/* Replaces the pybind `stroke_backward` of source/strokelib/src/strokes.h:38-54
+ * (kernel strokes_backward.cu:11-157, host :224-287), called from
+ * strokes.py:185-224.  grad_shape_params [S,P], grad_color_params [S,Q] and
+ * grad_x [N,3] (or NULL) are accumulated.  grad_sdf may be NULL.
+ * adjoint_exact = 1: the true derivative of the forward; 0: the reference's
+ * hand-written adjoint term for term. */
+int n3ds_stroke_backward(float* grad_shape_params, float* grad_color_params, float* grad_x,
+                         const float* grad_alpha, const float* grad_color, const float* grad_sdf,
+                         const float* x, const float* radius, const float* viewdir, const float* alpha,
+                         const float* shape_params, const float* color_params, int64_t n_points,
+                         int64_t n_viewdirs, int64_t n_strokes, int64_t n_shape_params,
+                         int64_t n_color_params, uint32_t sdf_id, uint32_t color_id, float sdf_delta,
+                         int use_laplace_transform, int inv_scale_radius, int adjoint_exact, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Ordered "over" compositing of per-stroke alpha / colour / density.
+ * Replaces the pybind `compose_forward` of
+ * source/strokelib/src/compositing.h:11-20 (kernel compositing.cu:9-59,
+ * host :159-204), called from strokes.py:316-346.
+ * density_out [N], color_out [N,D]; alphas [N,S], colors [N,S,D] with
+ * D in {1,3}, density_params [S].  Stroke S-1 is in front.
+ * ------------------------------------------------------------------------ */
+int n3ds_compose_forward(float* density_out, float* color_out, const float* alphas, const float* colors,
+                         const float* density_params, int64_t n_points, int64_t n_strokes, int64_t color_dim,
+                         void* stream);
+
+/* Replaces the pybind `compose_backward` of compositing.h:22-32 (kernel
+ * compositing.cu:61-157, host :206-263), called from strokes.py:348-373.
+ * grad_alphas [N,S] and grad_colors [N,S,D] are written; grad_density_params
+ * [S] is accumulated.  grad_density_out / grad_color_out may be NULL (zeros).
+ * n_strokes <= 8192. */
+int n3ds_compose_backward(float* grad_alphas, float* grad_colors, float* grad_density_params,
+                          const float* grad_density_out, const float* grad_color_out, const float* alphas,
+                          const float* colors, const float* density_params, int64_t n_points,
+                          int64_t n_strokes, int64_t color_dim, int adjoint_exact, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Multiresolution hash-grid encoder.
+ * Replaces the pybind `grid_encode_forward` of
+ * source/gridencoder/src/gridencoder.h:12 (kernel_grid gridencoder.cu:87-245,
+ * host :372-400, :448-471), called from grid.py:24-63.
+ * inputs [B,D] float32 in [0,1]; embeddings [n_rows,C] of `dtype`
+ * (0 = float32, 1 = float16); offsets [L+1] int32; outputs [B, L*C] of
+ * `dtype` (already in the [B, L*C] order that grid.py obtains by permuting
+ * [L,B,C]); S = log2(per_level_scale), H = base_resolution; dy_dx
+ * [B, L*D*C] of `dtype` or NULL.  gridtype 0 = hash, 1 = tiled; interp
+ * 0 = linear, 1 = smoothstep.  D in {2,3}, C in {1,2,4,8}; float16 tables
+ * are interpolated in float32.
+ * ------------------------------------------------------------------------ */
+int n3ds_grid_encode_forward(const float* inputs, const void* embeddings, const int32_t* offsets,
+                             void* outputs, int64_t B, int D, int C, int L, float S, int H, void* dy_dx,
+                             int gridtype, int align_corners, int interp, int dtype, void* stream);
+
+/* Replaces the pybind `grid_encode_backward` of gridencoder.h:13
+ * (kernel_grid_backward gridencoder.cu:248-340, kernel_input_backward
+ * :343-369, host :402-444, :473-503), called from grid.py:65-89.
+ * grad [B, L*C] of `dtype`; grad_embeddings_f32 [n_rows,C] float32 is
+ * accumulated (zero-initialised by the caller); grad_embeddings_out
+ * [n_rows,C] of `dtype` receives the rounded result when dtype = 1 (may be
+ * NULL); grad_inputs [B,D] (written) needs dy_dx, both may be NULL. */
+int n3ds_grid_encode_backward(const void* grad, const float* inputs, const int32_t* offsets,
+                              float* grad_embeddings_f32, void* grad_embeddings_out, int64_t n_rows, int64_t B,
+                              int D, int C, int L, float S, int H, const void* dy_dx, void* grad_inputs,
+                              int gridtype, int align_corners, int interp, int dtype, void* stream);
+
+/* Replaces the pybind `grad_total_variation` of gridencoder.h:15
+ * (kernel_grad_tv gridencoder.cu:506-645), called from grid.py:172-194.
+ * Adds weight * TV gradient into grad_f32 [n_rows,C] float32. */
+int n3ds_grid_grad_tv(const float* inputs, const void* embeddings, float* grad_f32, const int32_t* offsets,
+                      float weight, int64_t B, int D, int C, int L, float S, int H, int gridtype,
+                      int align_corners, int dtype, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Fused stroke-field renderer (one pass per ray: per-sample stroke SDF ->
+ * alpha / colour for a mixed stroke palette, ordered "over" compositing and
+ * the volume-render ray march).  It replaces, as one call, the chain of
+ * source/models.py:660-711 (StrokeField.predict_density: one stroke_forward
+ * per stroke type, columns by stroke index, compose_strokes) followed by
+ * source/utils/render.py:29-48 (compute_alpha_weights) and :51-116
+ * (volumetric_rendering), with its adjoint.
+ * ------------------------------------------------------------------------ */
+
+/* Floats of the stroke table for n_strokes strokes (host-side size query). */
+int64_t n3ds_render_table_floats(int n_strokes);
+
+/* Builds the per-stroke table (rotation, scale, culling bounds) from
+ * stroke_ids [S] int32 (one sdf_id per stroke) and shape_params [S,shape_width]
+ * (each row's leading num_shape_params(sdf_id) entries are used).  *status
+ * (device int, zero-initialised) becomes non-zero for an invalid id. */
+int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* shape_params, int n_strokes,
+                        int shape_width, int* status, void* stream);
+
+/* Forward.  origins/directions/viewdirs [R,3], radii [R], tdist [R,n+1]
+ * (sample k at the midpoint of tdist[k], tdist[k+1], as render.py:5-26),
+ * color_params [S,color_width], density_params [S]; n <= 256.
+ * Outputs rgb [R,3], acc [R], depth [R]; state_out [5, R*n] is the per-sample
+ * composite state the backward reads.  counters [25] uint64 (or NULL) receive
+ * per-base-type counts of evaluated and alpha>0 (sample, stroke) pairs.
+ * cone != 0: sample radius = radii * t; coord_div divides sample coordinates
+ * (models.py:663 warp); bg_color is a host pointer to 3 floats; cull_eps is the
+ * alpha below which the Laplace transfer's tail is culled (linear transfer:
+ * culling is exact). */
+int n3ds_render_forward(const float* origins, const float* directions, const float* viewdirs, const float* radii,
+                        const float* tdist, const float* table, const float* color_params,
+                        const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
+                        int shape_width, int color_width, int color_id, float sdf_delta,
+                        int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
+                        const float* bg_color, float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
+                        float* state_out, unsigned long long* counters, void* stream);
+
+/* Floats of device workspace n3ds_render_backward needs (queries the current
+ * device; -1 on a HIP error). */
+int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_strokes, int shape_width, int color_width);
+
+/* Backward: accumulates dL/d(shape_params) [S,shape_width], dL/d(color_params)
+ * [S,color_width] and dL/d(density_params) [S] for upstream grad_rgb [R,3],
+ * grad_acc [R] (or NULL) and grad_depth [R] (or NULL), recomputing the SDFs
+ * from the forward's `state`.  adjoint_exact as for n3ds_stroke_backward. */
+int n3ds_render_backward(const float* origins, const float* directions, const float* viewdirs, const float* radii,
+                         const float* tdist, const float* table, const float* color_params,
+                         const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
+                         int shape_width, int color_width, int color_id, float sdf_delta,
+                         int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
+                         const float* bg_color, float cull_eps, int adjoint_exact, const float* grad_rgb,
+                         const float* grad_acc, const float* grad_depth, const float* state, float* workspace,
+                         int64_t workspace_floats, float* grad_shape, float* grad_color, float* grad_density,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* N3DS_H */

@@ -18,7 +18,9 @@ SOURCES = ['n3ds_capi.cpp', 'n3ds_strokes.hip', 'n3ds_compose.hip', 'n3ds_grid.h
 ARCH = os.environ.get('N3DS_OFFLOAD_ARCH', 'gfx950')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 
-COMMON = ['-O3', '-std=c++17', '-fPIC', '-fvisibility=hidden', f'--offload-arch={ARCH}',
+INCLUDE = os.path.join(os.path.dirname(HERE), 'include')  # public C-ABI header n3ds.h
+
+COMMON = ['-O3', '-std=c++17', '-fPIC', '-fvisibility=hidden', f'--offload-arch={ARCH}', f'-I{INCLUDE}',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-result']
 
 

@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "n3ds.h"  // public declarations: every definition below must match them
+
 namespace n3ds {
 
 void set_error(const char* fmt, ...);

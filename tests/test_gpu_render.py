@@ -114,3 +114,25 @@ def test_render_matches_dropin_chain(dev):
     rgb2 = (w[..., None] * col).sum(-2) + (1 - acc2[:, None]).clamp_min(0)
     assert rel_err(rgb, rgb2) < 1e-5
     assert rel_err(acc, acc2) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', ['cfg0', 'cfg1'])
+def test_render_golden_fixture(case, dev):
+    """HIP renderer against the committed float64 fixture tests/golden/field_small.npz."""
+    import os
+
+    import numpy as np
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'field_small.npz'))
+    t = {k[len(case) + 1:]: torch.from_numpy(z[k]) for k in z.files if k.startswith(case + '_')}
+    sp = t['shape_params'].to(dev).requires_grad_(True)
+    cp = t['color_params'].to(dev).requires_grad_(True)
+    dp = t['density_params'].to(dev).requires_grad_(True)
+    rgb, acc, _ = render.render_strokes(*(t[k].to(dev) for k in ('origins', 'directions', 'viewdirs', 'radii',
+                                                                 'tdist')), t['stroke_ids'], sp, cp, dp)
+    ((rgb * t['g_rgb'].float().to(dev)).sum() + (acc * t['g_acc'].float().to(dev)).sum()).backward()
+    assert rel_err(rgb, t['rgb']) < 1e-5
+    assert rel_err(acc, t['acc']) < 1e-5
+    assert rel_err(sp.grad, t['grad_shape']) < 1e-4
+    assert rel_err(cp.grad, t['grad_color']) < 1e-4
+    assert rel_err(dp.grad, t['grad_density']) < 1e-4

@@ -215,7 +215,9 @@ def main():
                      'flops_per_launch': kernels[dom][0], 'ms_per_launch': kernels[dom][1]},
         'kernels_ms': {'render_fwd': fwd_ms, 'render_bwd': bwd_ms},
         'useful_pairs': {BASE_NAMES[b]: c for b, c in pos_pairs.items()},
-        'evaluated_pair_lanes': eval_lanes,
+        'evaluated_pairs': {BASE_NAMES[b]: cnt[12 + b] for b in range(12) if cnt[12 + b]},
+        'evaluated_pairs_total': eval_lanes,
+        'stroke_hits': cnt[24],
         'useful_gflop_per_step': useful_total / 1e9,
         'wall_s_timed': wall,
     }

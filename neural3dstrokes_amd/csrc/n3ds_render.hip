@@ -10,26 +10,39 @@
 // compute_alpha_weights / volumetric_rendering.
 //
 // Mapping (one wavefront per ray):
-//  * lane l owns samples l, l+64, ... (NS slots, n <= 64*NS);
+//  * lane l owns samples l, l+64, ... (NS slots, n <= 64*NS); per-sample
+//    transmittance / density / colour live in registers and the ray march is
+//    a wavefront scan;
 //  * strokes are visited in compositing order in chunks of 64: each lane
 //    tests one stroke against the ray (unit-space slab test for solids, the
-//    boxes of de Casteljau pieces for curves), the hits are compacted with a
-//    ballot and walked one by one with the stroke index broadcast by readlane,
-//    so the stroke record is fetched with scalar loads and the type switch is
-//    wave-uniform;
-//  * per-sample transmittance / density / colour live in registers; the ray
-//    march is a wavefront scan.
+//    boxes of Bezier-hull pieces for curves) and turns the hit interval into
+//    a range of sample indices (binary search over the sample depths in LDS);
+//  * the hits are compacted with a ballot and their (sample, stroke) pairs
+//    are laid out contiguously (wave prefix sum of the range lengths); the
+//    expensive per-pair work (SDF, alpha, and in the adjoint the SDF
+//    gradient and transform adjoint) runs one pair per lane, so a stroke that
+//    covers 5 of 128 samples costs 5 lanes, not a wavefront;
+//  * the order-dependent part (over-compositing and its adjoint) then walks
+//    the hit strokes in order, each lane updating its own samples from the
+//    per-pair values staged in LDS;
+//  * adjoint: per-pair gradient slots are reduced per stroke through an LDS
+//    transpose (each lane sums one (stroke, slot) segment) into per-block
+//    LDS accumulators, flushed once per persistent block.
 // Culling is exact for the linear alpha transfer (alpha = 0 outside the
 // tested volume); for the Laplace transfer it drops alpha below `cull_eps`.
+// Sample depths must be non-decreasing along a ray (as the reference's
+// stratified samplers produce); a ray whose depths are not sorted falls back
+// to evaluating every hit stroke on all its samples (same result, slower).
 #include "n3ds_capi.h"
 #include "n3ds_stroke_common.h"
 
 namespace n3ds {
 
-constexpr int REC = 64;     // floats per stroke record
-constexpr int NPIECE = 5;   // curve pieces bounded for culling
-constexpr int RWAVES = 4;   // wavefronts per block
-constexpr int GSLOTS = 32;  // canonical gradient slots per reduce round
+constexpr int REC = 64;      // floats per stroke record
+constexpr int NPIECE = 5;    // curve pieces bounded for culling
+constexpr int FWAVES = 4;    // wavefronts per forward block
+constexpr int BWAVES = 8;    // wavefronts per backward block (at most)
+constexpr int PBUF = 256;    // (sample, stroke) pairs staged per window (>= samples per ray)
 
 // record layout
 enum : int {
@@ -281,215 +294,333 @@ __device__ __forceinline__ V3 stroke_color_uniform(const float* __restrict__ cp,
     return v3s(0.f);
 }
 
-template <int NS>
-struct RayState {
-    V3 x[NS];
-    float r[NS];
-    float t[NS];
-    bool valid[NS];
+// ---------------------------------------------------------------------------
+// per-ray constants and the per-wavefront pair machinery
+// ---------------------------------------------------------------------------
+struct RayConst {
+    V3 o, d, vd, oc, dc;  // oc, dc: origin / direction in the (warped) stroke coordinates
+    float radii, tlo, thi, m;
 };
 
-template <int NS>
-__device__ __forceinline__ void load_ray(const RenderArgs& A, int64_t ray, int lane, RayState<NS>& rs, V3& o, V3& d,
-                                         V3& vd, float& tlo, float& thi, float& rmax) {
-    o = ld3(A.origins + 3 * ray);
-    d = ld3(A.dirs + 3 * ray);
-    vd = ld3(A.viewdirs + 3 * ray);
-    const float radii = A.radii[ray];
+__device__ __forceinline__ RayConst load_ray(const RenderArgs& A, int64_t ray) {
+    RayConst c;
+    c.o = ld3(A.origins + 3 * ray);
+    c.d = ld3(A.dirs + 3 * ray);
+    c.vd = ld3(A.viewdirs + 3 * ray);
+    c.radii = A.radii[ray];
     const float* td = A.tdist + ray * (A.n + 1);
-    tlo = 0.5f * (td[0] + td[1]);
-    thi = 0.5f * (td[A.n - 1] + td[A.n]);
-    rmax = A.cone ? radii * fmaxf(fabsf(tlo), fabsf(thi)) : radii;
+    c.tlo = 0.5f * (td[0] + td[1]);
+    c.thi = 0.5f * (td[A.n - 1] + td[A.n]);
+    const float rmax = A.cone ? c.radii * fmaxf(fabsf(c.tlo), fabsf(c.thi)) : c.radii;
+    c.m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
+    c.oc = c.o;
+    c.dc = c.d;
+    if (A.coord_div != 1.f) {
+        c.oc = v3(c.o.x / A.coord_div, c.o.y / A.coord_div, c.o.z / A.coord_div);
+        c.dc = v3(c.d.x / A.coord_div, c.d.y / A.coord_div, c.d.z / A.coord_div);
+    }
+    return c;
+}
+
+// sample position (stroke coordinates) and radius at depth t, render.py:5-26
+__device__ __forceinline__ void sample_point(const RenderArgs& A, const RayConst& c, float t, V3& x, float& r) {
+    V3 xw = c.d * t + c.o;
+    if (A.coord_div != 1.f) xw = v3(xw.x / A.coord_div, xw.y / A.coord_div, xw.z / A.coord_div);
+    x = xw;
+    r = A.cone ? c.radii * t : c.radii;
+}
+
+// LDS ordering between the lanes of one wavefront (no block barrier)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// per-wavefront LDS region
+struct WaveMem {
+    float* ts;   // [64*NS] sample depth, +inf past the last sample
+    int* ck;     // [64] stroke index of the c-th hit (compositing order)
+    int* ci0;    // [64] first sample of its interval
+    int* coff;   // [64] offset of its first pair within the chunk
+    int* ccnt;   // [64] its number of pairs
+    float* pb;   // [npf][PBUF] per-pair values of the current window
+    float* scr;  // [64][ng+1] per-lane gradient slots (adjoint)
+};
+
+__host__ __device__ constexpr int wave_mem_floats(int ns, int npf, int ng) {
+    return 64 * ns + 4 * 64 + npf * PBUF + (ng > 0 ? 64 * (ng + 1) : 0);
+}
+
+__device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf) {
+    WaveMem w;
+    w.ts = base;
+    int* ib = (int*)(base + 64 * ns);
+    w.ck = ib;
+    w.ci0 = ib + 64;
+    w.coff = ib + 128;
+    w.ccnt = ib + 192;
+    w.pb = base + 64 * ns + 256;
+    w.scr = w.pb + npf * PBUF;
+    return w;
+}
+
+// number of samples with depth < v (LE: <= v); ts is sorted with 64*NS entries
+template <int NS, bool LE>
+__device__ __forceinline__ int count_below(const float* ts, float v) {
+    int pos = 0;
+#pragma unroll
+    for (int step = 256; step > 0; step >>= 1) {
+        if (pos + step <= 64 * NS) {
+            const float t = ts[pos + step - 1];
+            if (LE ? (t <= v) : (t < v)) pos += step;
+        }
+    }
+    return pos;
+}
+
+// hit c in [c0, c1) whose pair range contains window-relative pair p
+__device__ __forceinline__ int find_hit(const WaveMem& w, int c0, int c1, int wbase, int p) {
+    int c = c0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int m = c + step;
+        if (m < c1 && w.coff[m] - wbase <= p) c = m;
+    }
+    return c;
+}
+
+// Writes the ray's sample depths to LDS and returns whether they are sorted.
+template <int NS>
+__device__ __forceinline__ bool stage_depths(const RenderArgs& A, int64_t ray, int lane, const WaveMem& w,
+                                             float (&t)[NS], bool (&valid)[NS]) {
+    const float* td = A.tdist + ray * (A.n + 1);
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        int i = lane + 64 * s;
-        rs.valid[s] = i < A.n;
-        int ii = rs.valid[s] ? i : 0;
-        float t = (td[ii] + td[ii + 1]) / 2.f;
-        rs.t[s] = t;
-        V3 xw = d * t + o;
-        if (A.coord_div != 1.f) xw = v3(xw.x / A.coord_div, xw.y / A.coord_div, xw.z / A.coord_div);
-        rs.x[s] = xw;
-        rs.r[s] = A.cone ? radii * t : radii;
+        const int i = lane + 64 * s;
+        valid[s] = i < A.n;
+        const int ii = valid[s] ? i : 0;
+        t[s] = (td[ii] + td[ii + 1]) / 2.f;
+        w.ts[i] = valid[s] ? t[s] : INFINITY;
     }
+    wave_sync();
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + 64 * s;
+        if (i + 1 < A.n && w.ts[i] > w.ts[i + 1]) bad = true;
+    }
+    return __ballot(bad) == 0ull;
+}
+
+// Culls the 64 strokes of the chunk starting at base0 (in compositing order)
+// and writes the compacted hits to LDS.  Returns the number of hits.
+template <int NS>
+__device__ __forceinline__ int collect_hits(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
+                                            int base0, int lane) {
+    const int k = A.S - 1 - (base0 + lane);
+    float ta = 0.f, tb = -1.f;
+    bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.m, A.isr, rc.tlo, rc.thi, ta, tb);
+    int i0 = 0, cnt = 0;
+    if (hit) {
+        if (sorted) {
+            i0 = count_below<NS, false>(w.ts, ta);
+            cnt = count_below<NS, true>(w.ts, tb) - i0;
+        } else {
+            cnt = A.n;
+        }
+        hit = cnt > 0;
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int incl = wave_incl_scan(hit ? cnt : 0, lane);
+    if (hit) {
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        w.ck[rank] = k;
+        w.ci0[rank] = i0;
+        w.coff[rank] = incl - cnt;
+        w.ccnt[rank] = cnt;
+    }
+    wave_sync();
+    return __popcll(mask);
+}
+
+// Largest window [c0, c1) of hits whose pairs fit the staging buffer.
+__device__ __forceinline__ int window_end(const WaveMem& w, int c0, int nc, int wbase, int lane) {
+    const bool fits = lane >= c0 && lane < nc && w.coff[lane] + w.ccnt[lane] - wbase <= PBUF;
+    return c0 + __popcll(__ballot(fits));
 }
 
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
 template <int NS, int CID>
-__global__ __launch_bounds__(64 * RWAVES) void render_fwd_kernel(RenderArgs A, float* __restrict__ rgb_out,
+__global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, float* __restrict__ rgb_out,
                                                                  float* __restrict__ acc_out,
                                                                  float* __restrict__ depth_out,
                                                                  float* __restrict__ state,
                                                                  unsigned long long* __restrict__ counters) {
-    __shared__ unsigned long long cnt[2 * NUM_BASE + 1];
+    extern __shared__ float smem[];
+    constexpr int NPF = CID == 1 ? 4 : 1;  // alpha (+ colour for gradient_rgb)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (counters) {
-        for (int i = threadIdx.x; i < 2 * NUM_BASE + 1; i += blockDim.x) cnt[i] = 0ull;
-        __syncthreads();
-    }
-    const int64_t ray = (int64_t)blockIdx.x * RWAVES + wave;
-    if (ray < A.R) {
-        RayState<NS> rs;
-        V3 o, d, vd;
-        float tlo, thi, rmax;
-        load_ray<NS>(A, ray, lane, rs, o, d, vd, tlo, thi, rmax);
-        const float m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
-        V3 oc = o, dc = d;
-        if (A.coord_div != 1.f) {
-            oc = v3(o.x / A.coord_div, o.y / A.coord_div, o.z / A.coord_div);
-            dc = v3(d.x / A.coord_div, d.y / A.coord_div, d.z / A.coord_div);
-        }
-        float T[NS], Dn[NS];
-        V3 C[NS];
+    const int64_t ray = (int64_t)blockIdx.x * FWAVES + wave;
+    if (ray >= A.R) return;
+    const WaveMem w = wave_mem(smem + wave * wave_mem_floats(NS, NPF, 0), NS, NPF);
+    const RayConst rc = load_ray(A, ray);
+    float ts[NS];
+    bool valid[NS];
+    const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
+    float T[NS], Dn[NS];
+    V3 C[NS];
 #pragma unroll
-        for (int s = 0; s < NS; s++) { T[s] = 1.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
-        for (int base0 = 0; base0 < A.S; base0 += 64) {
-            int k = A.S - 1 - (base0 + lane);
-            float ta = 0.f, tb = -1.f;
-            bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, oc, dc, m, A.isr, tlo, thi, ta, tb);
-            unsigned long long mask = __ballot(hit);
-            while (mask) {
-                int j = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int ks = __builtin_amdgcn_readlane(k, j);
-                const float sa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ta), j));
-                const float sb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tb), j));
-                const float* rec = A.table + (int64_t)ks * REC;
-                const int base = ibits(rec[R_BASE]);
-                const bool curve = ibits(rec[R_CURVE]) != 0;
-                const float* cp = A.cparams + (int64_t)ks * A.Q;
-                const float dk = A.dparams[ks];
-                const V3 cu = stroke_color_uniform<CID>(cp, vd);
-                bool fpos[NS], fev[NS];
+    for (int s = 0; s < NS; s++) { T[s] = 1.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
+    unsigned long long my_pos = 0, my_ev = 0, my_vis = 0;  // lane b counts stroke type b (counting pass)
+    for (int base0 = 0; base0 < A.S; base0 += 64) {
+        const int nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
+        for (int c0 = 0; c0 < nc;) {
+            const int wbase = w.coff[c0];
+            const int c1 = window_end(w, c0, nc, wbase, lane);
+            const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
+            // alpha of every (sample, stroke) pair of the window, one pair per lane
+            for (int p0 = 0; p0 < wp; p0 += 64) {
+                const int p = p0 + lane;
+                if (p < wp) {
+                    const int c = find_hit(w, c0, c1, wbase, p);
+                    const int k = w.ck[c];
+                    const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
+                    V3 x;
+                    float r;
+                    sample_point(A, rc, w.ts[i], x, r);
+                    const float* rec = A.table + (int64_t)k * REC;
+                    const Pair q = eval_pair(rec, ibits(rec[R_BASE]), ibits(rec[R_CURVE]) != 0, x, r, A);
+                    w.pb[p] = q.alpha;
+                    if (CID == 1) {
+                        const V3 col = color_eval_t<1>(q.p, rc.vd, A.cparams + (int64_t)k * A.Q);
+                        w.pb[PBUF + p] = col.x;
+                        w.pb[2 * PBUF + p] = col.y;
+                        w.pb[3 * PBUF + p] = col.z;
+                    }
+                }
+            }
+            wave_sync();
+            // ordered over-compositing, strokes front to back
+            for (int c = c0; c < c1; c++) {
+                const int k = __builtin_amdgcn_readfirstlane(w.ck[c]);
+                const int ci = __builtin_amdgcn_readfirstlane(w.ci0[c]);
+                const int cn = __builtin_amdgcn_readfirstlane(w.ccnt[c]);
+                const int po = __builtin_amdgcn_readfirstlane(w.coff[c] - wbase);
+                const float dk = A.dparams[k];
+                const V3 cu = stroke_color_uniform<CID>(A.cparams + (int64_t)k * A.Q, rc.vd);
+                int tp = 0;
 #pragma unroll
                 for (int s = 0; s < NS; s++) {
-                    bool act = rs.valid[s] && rs.t[s] >= sa && rs.t[s] <= sb;
-                    fpos[s] = false;
-                    fev[s] = act;
-                    if (act) {
-                        Pair q = eval_pair(rec, base, curve, rs.x[s], rs.r[s], A);
-                        if (q.alpha != 0.f) {
-                            V3 c = (CID == 1) ? color_eval_t<1>(q.p, vd, cp) : cu;
-                            float w = q.alpha * T[s];
-                            T[s] *= (1.f - q.alpha);
-                            Dn[s] += dk * w;
-                            C[s] = C[s] + c * w;
+                    const int j = lane + 64 * s - ci;
+                    bool pos = false;
+                    if ((unsigned)j < (unsigned)cn) {
+                        const int q = po + j;
+                        const float a = w.pb[q];
+                        if (a != 0.f) {
+                            const V3 col = (CID == 1) ? v3(w.pb[PBUF + q], w.pb[2 * PBUF + q], w.pb[3 * PBUF + q]) : cu;
+                            const float wt = a * T[s];
+                            T[s] *= (1.f - a);
+                            Dn[s] += dk * wt;
+                            C[s] = C[s] + col * wt;
                         }
-                        fpos[s] = q.alpha > 0.f;
+                        pos = a > 0.f;
                     }
+                    if (counters) tp += __popcll(__ballot(pos));
                 }
                 if (counters) {
-                    int tp = 0, te = 0;
-#pragma unroll
-                    for (int s = 0; s < NS; s++) {
-                        tp += __popcll(__ballot(fpos[s]));
-                        te += __popcll(__ballot(fev[s]));
-                    }
-                    if (lane == 0) {
-                        atomicAdd(&cnt[base], (unsigned long long)tp);
-                        atomicAdd(&cnt[NUM_BASE + base], (unsigned long long)te);
-                        atomicAdd(&cnt[2 * NUM_BASE], 1ull);
-                    }
+                    const int base = ibits(A.table[(int64_t)k * REC + R_BASE]);
+                    if (lane == base) { my_pos += tp; my_ev += cn; my_vis += 1; }
                 }
             }
-        }
-        // save per-sample composite state and run the ray march
-        const int64_t N = A.R * (int64_t)A.n;
-        const float* td = A.tdist + ray * (A.n + 1);
-        const float dn = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
-        float sd[NS], rgbn[NS][3], wgt[NS];
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            int i = lane + 64 * s;
-            float scl = 1.f / (1.f + 1e-6f - T[s]);
-            rgbn[s][0] = clampf(C[s].x * scl, 0.f, 1.f);
-            rgbn[s][1] = clampf(C[s].y * scl, 0.f, 1.f);
-            rgbn[s][2] = clampf(C[s].z * scl, 0.f, 1.f);
-            if (rs.valid[s]) {
-                int64_t si = ray * A.n + i;
-                state[si] = Dn[s];
-                state[N + si] = C[s].x;
-                state[2 * N + si] = C[s].y;
-                state[3 * N + si] = C[s].z;
-                state[4 * N + si] = T[s];
-                sd[s] = Dn[s] * ((td[i + 1] - td[i]) * dn);
-            } else {
-                sd[s] = 0.f;
-            }
-        }
-        // exclusive prefix sum of density*delta over samples (slot-major order)
-        float carry = 0.f;
-        float acc = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, dsum = 0.f;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            float v = sd[s];
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                float u = __shfl_up(v, off, 64);
-                if (lane >= off) v += u;
-            }
-            float excl = carry + v - sd[s];
-            float trans = expf(-excl);
-            float al = 1.f - expf(-sd[s]);
-            float w = rs.valid[s] ? al * trans : 0.f;
-            wgt[s] = w;
-            carry += __shfl(v, 63, 64);
-            acc += w;
-            r0 += w * rgbn[s][0];
-            r1 += w * rgbn[s][1];
-            r2 += w * rgbn[s][2];
-            dsum += w * rs.t[s];
-        }
-        acc = wave_sum(acc);
-        r0 = wave_sum(r0);
-        r1 = wave_sum(r1);
-        r2 = wave_sum(r2);
-        dsum = wave_sum(dsum);
-        if (lane == 0) {
-            float bgw = fmaxf(1.f - acc, 0.f);
-            rgb_out[3 * ray + 0] = r0 + bgw * A.bg0;
-            rgb_out[3 * ray + 1] = r1 + bgw * A.bg1;
-            rgb_out[3 * ray + 2] = r2 + bgw * A.bg2;
-            acc_out[ray] = acc;
-            float dep = dsum / fmaxf(acc, 1.1920929e-07f);
-            if (isnan(dep)) dep = INFINITY;
-            depth_out[ray] = fminf(fmaxf(dep, td[0]), td[A.n]);
+            wave_sync();
+            c0 = c1;
         }
     }
-    if (counters) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < 2 * NUM_BASE + 1; i += blockDim.x)
-            if (cnt[i]) atomicAdd(counters + i, cnt[i]);
+    if (counters && lane < NUM_BASE) {
+        if (my_pos) atomicAdd(counters + lane, my_pos);
+        if (my_ev) atomicAdd(counters + NUM_BASE + lane, my_ev);
+        if (my_vis) atomicAdd(counters + 2 * NUM_BASE, my_vis);
+    }
+    // save per-sample composite state and run the ray march
+    const int64_t N = A.R * (int64_t)A.n;
+    const float* td = A.tdist + ray * (A.n + 1);
+    const float dn = sqrtf(rc.d.x * rc.d.x + rc.d.y * rc.d.y + rc.d.z * rc.d.z);
+    float sd[NS], rgbn[NS][3];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + 64 * s;
+        const float scl = 1.f / (1.f + 1e-6f - T[s]);
+        rgbn[s][0] = clampf(C[s].x * scl, 0.f, 1.f);
+        rgbn[s][1] = clampf(C[s].y * scl, 0.f, 1.f);
+        rgbn[s][2] = clampf(C[s].z * scl, 0.f, 1.f);
+        if (valid[s]) {
+            const int64_t si = ray * A.n + i;
+            state[si] = Dn[s];
+            state[N + si] = C[s].x;
+            state[2 * N + si] = C[s].y;
+            state[3 * N + si] = C[s].z;
+            state[4 * N + si] = T[s];
+            sd[s] = Dn[s] * ((td[i + 1] - td[i]) * dn);
+        } else {
+            sd[s] = 0.f;
+        }
+    }
+    // exclusive prefix sum of density*delta over samples (slot-major order)
+    float carry = 0.f;
+    float acc = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, dsum = 0.f;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        float v = sd[s];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const float u = __shfl_up(v, off, 64);
+            if (lane >= off) v += u;
+        }
+        const float excl = carry + v - sd[s];
+        const float trans = expf(-excl);
+        const float al = 1.f - expf(-sd[s]);
+        const float wt = valid[s] ? al * trans : 0.f;
+        carry += __shfl(v, 63, 64);
+        acc += wt;
+        r0 += wt * rgbn[s][0];
+        r1 += wt * rgbn[s][1];
+        r2 += wt * rgbn[s][2];
+        dsum += wt * ts[s];
+    }
+    acc = wave_sum(acc);
+    r0 = wave_sum(r0);
+    r1 = wave_sum(r1);
+    r2 = wave_sum(r2);
+    dsum = wave_sum(dsum);
+    if (lane == 0) {
+        const float bgw = fmaxf(1.f - acc, 0.f);
+        rgb_out[3 * ray + 0] = r0 + bgw * A.bg0;
+        rgb_out[3 * ray + 1] = r1 + bgw * A.bg1;
+        rgb_out[3 * ray + 2] = r2 + bgw * A.bg2;
+        acc_out[ray] = acc;
+        float dep = dsum / fmaxf(acc, 1.1920929e-07f);
+        if (isnan(dep)) dep = INFINITY;
+        depth_out[ray] = fminf(fmaxf(dep, td[0]), td[A.n]);
     }
 }
 
 // ---------------------------------------------------------------------------
 // backward
 // ---------------------------------------------------------------------------
-// Reduce-scatter of 32 per-lane values v[OFF .. OFF+32) across the wavefront:
-// afterwards lane l holds the wavefront sum of value OFF + (l >> 1).  Each step
-// exchanges half of the remaining values with the partner lane, so 32 values
-// cost 16+8+4+2+1+1 shuffles instead of 32*6.  Destroys v[OFF..OFF+16).
-template <int OFF, int N>
-__device__ __forceinline__ float reduce_scatter32(float (&v)[N], int lane) {
-#pragma unroll
-    for (int step = 0; step < 5; step++) {
-        const int half = 16 >> step;           // values kept after this step
-        const int mask = 32 >> step;           // partner lane distance
-        const bool upper = (lane & mask) != 0;
-#pragma unroll
-        for (int i = 0; i < half; i++) {
-            float keep = upper ? v[OFF + half + i] : v[OFF + i];
-            float send = upper ? v[OFF + i] : v[OFF + half + i];
-            v[OFF + i] = keep + __shfl_xor(send, mask, 64);
-        }
-    }
-    return v[OFF] + __shfl_xor(v[OFF], 1, 64);
-}
-
 // canonical gradient slot -> column in the [Pmax | Q | 1] accumulator row (-1: none)
+// slots: 0..13 shape (solids: 0-1 base, 2-4 scale, 5-7 euler, 8-10 translation;
+// curves: their 11 or 14 params), 14..14+Q-1 colour, 14+Q density
 __device__ __forceinline__ int slot_column(int slot, const float* __restrict__ rec, int Pmax, int Q) {
     const bool curve = ibits(rec[R_CURVE]) != 0;
     const int nb = ibits(rec[R_NB]);
@@ -520,44 +651,43 @@ __device__ __forceinline__ int slot_column(int slot, const float* __restrict__ r
 }
 
 template <int NS, int CID>
-__global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, const float* __restrict__ g_rgb,
+__global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, const float* __restrict__ g_rgb,
                                                                  const float* __restrict__ g_acc,
                                                                  const float* __restrict__ g_depth,
                                                                  const float* __restrict__ state,
                                                                  float* __restrict__ partials, int use_lds,
                                                                  float* __restrict__ gdirect) {
-    extern __shared__ float gacc[];  // [S * PG] when use_lds
+    extern __shared__ float smem[];  // [S * PG] accumulators (use_lds) + per-wavefront regions
+    constexpr int NQ = num_color_params(CID);
+    constexpr int NG = 15 + NQ;  // gradient slots per pair
+    constexpr int NPF = 5;       // per-pair values: alpha -> dL/dalpha, colour / dL/d(density, colour)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = blockDim.x >> 6;
     const int PG = A.Pmax + A.Q + 1;
-    const int64_t nacc = (int64_t)A.S * PG;
-    if (use_lds) {
-        for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) gacc[i] = 0.f;
-        __syncthreads();
-    }
+    const int64_t nacc = use_lds ? (int64_t)A.S * PG : 0;
+    float* gacc = smem;
+    float* dst = use_lds ? gacc : gdirect;
+    for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) gacc[i] = 0.f;
+    __syncthreads();
+    const WaveMem w = wave_mem(smem + nacc + wave * wave_mem_floats(NS, NPF, NG), NS, NPF);
     const int64_t N = A.R * (int64_t)A.n;
-    const int64_t nwaves = (int64_t)gridDim.x * RWAVES;
-    for (int64_t ray = (int64_t)blockIdx.x * RWAVES + wave; ray < A.R; ray += nwaves) {
-        RayState<NS> rs;
-        V3 o, d, vd;
-        float tlo, thi, rmax;
-        load_ray<NS>(A, ray, lane, rs, o, d, vd, tlo, thi, rmax);
-        const float m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
-        V3 oc = o, dc = d;
-        if (A.coord_div != 1.f) {
-            oc = v3(o.x / A.coord_div, o.y / A.coord_div, o.z / A.coord_div);
-            dc = v3(d.x / A.coord_div, d.y / A.coord_div, d.z / A.coord_div);
-        }
+    const int64_t nwaves = (int64_t)gridDim.x * W;
+    for (int64_t ray = (int64_t)blockIdx.x * W + wave; ray < A.R; ray += nwaves) {
+        const RayConst rc = load_ray(A, ray);
+        float ts[NS];
+        bool valid[NS];
+        const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
         const float* td = A.tdist + ray * (A.n + 1);
-        const float dn = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+        const float dn = sqrtf(rc.d.x * rc.d.x + rc.d.y * rc.d.y + rc.d.z * rc.d.z);
         // ---- ray-march adjoint from the saved per-sample state
         float Dt[NS], Tn[NS], sdl[NS], delta_s[NS];
         V3 Ct[NS], cn[NS];
         bool inr[NS][3];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            int i = lane + 64 * s;
-            if (rs.valid[s]) {
-                int64_t si = ray * A.n + i;
+            const int i = lane + 64 * s;
+            if (valid[s]) {
+                const int64_t si = ray * A.n + i;
                 Dt[s] = state[si];
                 Ct[s] = v3(state[N + si], state[2 * N + si], state[3 * N + si]);
                 Tn[s] = state[4 * N + si];
@@ -565,8 +695,8 @@ __global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, c
             } else {
                 Dt[s] = 0.f; Ct[s] = v3s(0.f); Tn[s] = 1.f; delta_s[s] = 0.f;
             }
-            float scl = 1.f / (1.f + 1e-6f - Tn[s]);
-            V3 sv = Ct[s] * scl;
+            const float scl = 1.f / (1.f + 1e-6f - Tn[s]);
+            const V3 sv = Ct[s] * scl;
             inr[s][0] = 0.f <= sv.x && sv.x <= 1.f;
             inr[s][1] = 0.f <= sv.y && sv.y <= 1.f;
             inr[s][2] = 0.f <= sv.z && sv.z <= 1.f;
@@ -580,16 +710,16 @@ __global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, c
             float v = sdl[s];
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
-                float u = __shfl_up(v, off, 64);
+                const float u = __shfl_up(v, off, 64);
                 if (lane >= off) v += u;
             }
-            float excl = carry + v - sdl[s];
+            const float excl = carry + v - sdl[s];
             trv[s] = expf(-excl);
-            float al = 1.f - expf(-sdl[s]);
-            wv[s] = rs.valid[s] ? al * trv[s] : 0.f;
+            const float al = 1.f - expf(-sdl[s]);
+            wv[s] = valid[s] ? al * trv[s] : 0.f;
             carry += __shfl(v, 63, 64);
             acc += wv[s];
-            dsum += wv[s] * rs.t[s];
+            dsum += wv[s] * ts[s];
         }
         acc = wave_sum(acc);
         dsum = wave_sum(dsum);
@@ -605,8 +735,8 @@ __global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, c
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             gw[s] = dot(grgb, cn[s]) + gacc_r - bgdot;
-            if (dep_in) gw[s] += gdep * (acc > 1.1920929e-07f ? (rs.t[s] - dep) / accc : rs.t[s] / accc);
-            if (!rs.valid[s]) gw[s] = 0.f;
+            if (dep_in) gw[s] += gdep * (acc > 1.1920929e-07f ? (ts[s] - dep) / accc : ts[s] / accc);
+            if (!valid[s]) gw[s] = 0.f;
         }
         // suffix (exclusive) sums of w*gw over samples after this one
 #pragma unroll
@@ -614,11 +744,11 @@ __global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, c
             float v = wv[s] * gw[s];
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
-                float u = __shfl_down(v, off, 64);
+                const float u = __shfl_down(v, off, 64);
                 if (lane + off < 64) v += u;
             }
-            float excl = suf_in + v - wv[s] * gw[s];
-            float tnext = trv[s] * expf(-sdl[s]);
+            const float excl = suf_in + v - wv[s] * gw[s];
+            const float tnext = trv[s] * expf(-sdl[s]);
             gsig[s] = delta_s[s] * (tnext * gw[s] - excl);
             suf_in += __shfl(v, 0, 64);
         }
@@ -627,119 +757,176 @@ __global__ __launch_bounds__(64 * RWAVES) void render_bwd_kernel(RenderArgs A, c
         V3 gC[NS], C2[NS];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            gD[s] = rs.valid[s] ? gsig[s] : 0.f;
-            V3 gcv = grgb * wv[s];
+            gD[s] = valid[s] ? gsig[s] : 0.f;
+            const V3 gcv = grgb * wv[s];
             gC[s] = v3(inr[s][0] ? gcv.x : 0.f, inr[s][1] ? gcv.y : 0.f, inr[s][2] ? gcv.z : 0.f);
             O[s] = 1.f + 1e-6f - Tn[s];
             Tp[s] = 1.f;
             D2[s] = 0.f;
             C2[s] = v3s(0.f);
         }
-        // ---- stroke loop in compositing order
+        // ---- strokes in compositing order
         for (int base0 = 0; base0 < A.S; base0 += 64) {
-            int k = A.S - 1 - (base0 + lane);
-            float ta = 0.f, tb = -1.f;
-            bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, oc, dc, m, A.isr, tlo, thi, ta, tb);
-            unsigned long long mask = __ballot(hit);
-            while (mask) {
-                int j = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int ks = __builtin_amdgcn_readlane(k, j);
-                const float sa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ta), j));
-                const float sb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tb), j));
-                const float* rec = A.table + (int64_t)ks * REC;
-                const int base = ibits(rec[R_BASE]);
-                const bool curve = ibits(rec[R_CURVE]) != 0;
-                const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
-                const float* cp = A.cparams + (int64_t)ks * A.Q;
-                const float dk = A.dparams[ks];
-                const V3 cu = stroke_color_uniform<CID>(cp, vd);
-                // canonical slots: 0..13 shape, 14..14+Q-1 colour, 14+Q density
-                constexpr int NQ = num_color_params(CID);
-                constexpr int GTOT = (15 + NQ) <= GSLOTS ? GSLOTS : 2 * GSLOTS;
-                float G[GTOT];
-#pragma unroll
-                for (int g = 0; g < GTOT; g++) G[g] = 0.f;
-                bool any = false;
-#pragma unroll
-                for (int s = 0; s < NS; s++) {
-                    bool act = rs.valid[s] && rs.t[s] >= sa && rs.t[s] <= sb;
-                    if (!act) continue;
-                    Pair q = eval_pair(rec, base, curve, rs.x[s], rs.r[s], A);
-                    const float a = q.alpha;
-                    if (a == 0.f) continue;
-                    any = true;
-                    V3 c = (CID == 1) ? color_eval_t<1>(q.p, vd, cp) : cu;
-                    const float Tk = Tp[s];
-                    const float w = a * Tk;
-                    const float Ta = Tk * (1.f - a);
-                    D2[s] += dk * w;
-                    C2[s] = C2[s] + c * w;
-                    const float sufD = Dt[s] - D2[s];
-                    const V3 sufC = Ct[s] - C2[s];
-                    const float inv1 = 1.f / fmaxf(1.f - a, 1e-6f);
-                    const float iO = 1.f / O[s];
-                    float ga;
-                    if (A.exact) {
-                        ga = gD[s] * (dk * Tk - sufD * inv1);
-                        ga += gC[s].x * ((c.x * Tk - sufC.x * inv1) * iO - Ct[s].x * (Tn[s] * inv1) * iO * iO);
-                        ga += gC[s].y * ((c.y * Tk - sufC.y * inv1) * iO - Ct[s].y * (Tn[s] * inv1) * iO * iO);
-                        ga += gC[s].z * ((c.z * Tk - sufC.z * inv1) * iO - Ct[s].z * (Tn[s] * inv1) * iO * iO);
-                    } else {
-                        ga = gD[s] * (Ta * dk - sufD) * inv1;
-                        float s2 = inv1 * iO * iO;
-                        ga += gC[s].x * (Ta * c.x * (O[s] - a) - sufC.x) * s2;
-                        ga += gC[s].y * (Ta * c.y * (O[s] - a) - sufC.y) * s2;
-                        ga += gC[s].z * (Ta * c.z * (O[s] - a) - sufC.z) * s2;
-                    }
-                    Tp[s] = Ta;
-                    // colour and density parameter gradients
-                    V3 gcol = gC[s] * (w * iO);
-                    V3 gpos = color_grad_t<CID>(q.p, vd, cp, gcol, G + 14, A.exact != 0);
-                    G[14 + NQ] += gD[s] * w;
-                    // shape gradients through alpha
-                    const float gs = ga * dalpha_dsdf(q.sdf, a, q.sc, A.delta, A.laplace != 0, A.exact != 0);
-                    const float* prm = curve ? rec + R_CP : rec + R_BP;
-                    V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
-                    gpos = gpos + dsdp * gs;
-                    if (!curve) {
-                        float gs1 = 0.f;
-                        V3 gs3 = v3s(0.f), gang = v3s(0.f), gtr = v3s(0.f);
-                        Frame F;
-                        F.trans = ld3(rec + R_TR);
-                        F.angle = ld3(rec + R_ANG);
-                        F.scale = ld3(rec + R_SC);
-#pragma unroll
-                        for (int ii = 0; ii < 3; ii++)
-#pragma unroll
-                            for (int jj = 0; jj < 3; jj++) F.R.r[ii][jj] = rec[R_ROT + 3 * ii + jj];
-                        if (A.exact && A.isr && (flags & (F_SS | F_MS))) {
-                            float gsc = ga * dalpha_dscale(q.sdf, a, q.sc, A.delta, A.laplace != 0);
-                            if (flags & F_SS) gs1 += gsc * q.sc / F.scale.x;
-                            else radius_factor_adjoint(base, q.p, prm, F.scale, gsc * q.sc / q.fac, gpos, gs3, G);
+            const int nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
+            for (int c0 = 0; c0 < nc;) {
+                const int wbase = w.coff[c0];
+                const int c1 = window_end(w, c0, nc, wbase, lane);
+                const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
+                // (1) alpha (and gradient_rgb colour) per pair
+                for (int p0 = 0; p0 < wp; p0 += 64) {
+                    const int p = p0 + lane;
+                    if (p < wp) {
+                        const int c = find_hit(w, c0, c1, wbase, p);
+                        const int k = w.ck[c];
+                        const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
+                        V3 x;
+                        float r;
+                        sample_point(A, rc, w.ts[i], x, r);
+                        const float* rec = A.table + (int64_t)k * REC;
+                        const Pair q = eval_pair(rec, ibits(rec[R_BASE]), ibits(rec[R_CURVE]) != 0, x, r, A);
+                        w.pb[p] = q.alpha;
+                        if (CID == 1) {
+                            const V3 col = color_eval_t<1>(q.p, rc.vd, A.cparams + (int64_t)k * A.Q);
+                            w.pb[PBUF + p] = col.x;
+                            w.pb[2 * PBUF + p] = col.y;
+                            w.pb[3 * PBUF + p] = col.z;
                         }
-                        xform_adjoint(gpos, q.p, rs.x[s], F, flags, gs1, gs3, gang, gtr);
-                        if (flags & F_SS) G[2] += gs1;
-                        else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
-                        G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
-                        G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
                     }
                 }
-                // wave reduction and accumulation
-                if (__ballot(any) == 0ull) continue;
-                float* dst = use_lds ? gacc : gdirect;
-                float v = reduce_scatter32<0>(G, lane);
-                if ((lane & 1) == 0) {
-                    int col = slot_column(lane >> 1, rec, A.Pmax, A.Q);
-                    if (col >= 0 && v != 0.f) atomicAdd(&dst[(int64_t)ks * PG + col], v);
-                }
-                if (GTOT > GSLOTS) {
-                    float v2 = reduce_scatter32<GTOT - GSLOTS>(G, lane);
-                    if ((lane & 1) == 0) {
-                        int col = slot_column(GSLOTS + (lane >> 1), rec, A.Pmax, A.Q);
-                        if (col >= 0 && v2 != 0.f) atomicAdd(&dst[(int64_t)ks * PG + col], v2);
+                wave_sync();
+                // (2) compositing adjoint, strokes front to back; per pair it leaves
+                //     dL/dalpha, dL/d(density param) and dL/d(colour) in the staging buffer
+                for (int c = c0; c < c1; c++) {
+                    const int k = __builtin_amdgcn_readfirstlane(w.ck[c]);
+                    const int ci = __builtin_amdgcn_readfirstlane(w.ci0[c]);
+                    const int cnn = __builtin_amdgcn_readfirstlane(w.ccnt[c]);
+                    const int po = __builtin_amdgcn_readfirstlane(w.coff[c] - wbase);
+                    const float dk = A.dparams[k];
+                    const V3 cu = stroke_color_uniform<CID>(A.cparams + (int64_t)k * A.Q, rc.vd);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) {
+                        const int j = lane + 64 * s - ci;
+                        if ((unsigned)j >= (unsigned)cnn) continue;
+                        const int q = po + j;
+                        const float a = w.pb[q];
+                        float ga = 0.f, gdw = 0.f;
+                        V3 gcol = v3s(0.f);
+                        if (a != 0.f) {
+                            const V3 c = (CID == 1) ? v3(w.pb[PBUF + q], w.pb[2 * PBUF + q], w.pb[3 * PBUF + q]) : cu;
+                            const float Tk = Tp[s];
+                            const float wt = a * Tk;
+                            const float Ta = Tk * (1.f - a);
+                            D2[s] += dk * wt;
+                            C2[s] = C2[s] + c * wt;
+                            const float sufD = Dt[s] - D2[s];
+                            const V3 sufC = Ct[s] - C2[s];
+                            const float inv1 = 1.f / fmaxf(1.f - a, 1e-6f);
+                            const float iO = 1.f / O[s];
+                            if (A.exact) {
+                                ga = gD[s] * (dk * Tk - sufD * inv1);
+                                ga += gC[s].x * ((c.x * Tk - sufC.x * inv1) * iO - Ct[s].x * (Tn[s] * inv1) * iO * iO);
+                                ga += gC[s].y * ((c.y * Tk - sufC.y * inv1) * iO - Ct[s].y * (Tn[s] * inv1) * iO * iO);
+                                ga += gC[s].z * ((c.z * Tk - sufC.z * inv1) * iO - Ct[s].z * (Tn[s] * inv1) * iO * iO);
+                            } else {
+                                ga = gD[s] * (Ta * dk - sufD) * inv1;
+                                const float s2 = inv1 * iO * iO;
+                                ga += gC[s].x * (Ta * c.x * (O[s] - a) - sufC.x) * s2;
+                                ga += gC[s].y * (Ta * c.y * (O[s] - a) - sufC.y) * s2;
+                                ga += gC[s].z * (Ta * c.z * (O[s] - a) - sufC.z) * s2;
+                            }
+                            Tp[s] = Ta;
+                            gdw = gD[s] * wt;
+                            gcol = gC[s] * (wt * iO);
+                        }
+                        w.pb[q] = ga;
+                        w.pb[PBUF + q] = gdw;
+                        w.pb[2 * PBUF + q] = gcol.x;
+                        w.pb[3 * PBUF + q] = gcol.y;
+                        w.pb[4 * PBUF + q] = gcol.z;
                     }
                 }
+                wave_sync();
+                // (3) parameter gradients per pair, reduced per stroke
+                for (int p0 = 0; p0 < wp; p0 += 64) {
+                    const int p = p0 + lane;
+                    float G[NG];
+#pragma unroll
+                    for (int g = 0; g < NG; g++) G[g] = 0.f;
+                    int c = c0;
+                    if (p < wp) {
+                        c = find_hit(w, c0, c1, wbase, p);
+                        const float ga = w.pb[p];
+                        const float gdw = w.pb[PBUF + p];
+                        const V3 gcol = v3(w.pb[2 * PBUF + p], w.pb[3 * PBUF + p], w.pb[4 * PBUF + p]);
+                        if (ga != 0.f || gdw != 0.f || gcol.x != 0.f || gcol.y != 0.f || gcol.z != 0.f) {
+                            const int k = w.ck[c];
+                            const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
+                            V3 x;
+                            float r;
+                            sample_point(A, rc, w.ts[i], x, r);
+                            const float* rec = A.table + (int64_t)k * REC;
+                            const int base = ibits(rec[R_BASE]);
+                            const bool curve = ibits(rec[R_CURVE]) != 0;
+                            const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
+                            const float* cp = A.cparams + (int64_t)k * A.Q;
+                            const Pair q = eval_pair(rec, base, curve, x, r, A);
+                            V3 gpos = color_grad_t<CID>(q.p, rc.vd, cp, gcol, G + 14, A.exact != 0);
+                            G[14 + NQ] += gdw;
+                            const float gs = ga * dalpha_dsdf(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0,
+                                                              A.exact != 0);
+                            const float* prm = curve ? rec + R_CP : rec + R_BP;
+                            const V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
+                            gpos = gpos + dsdp * gs;
+                            if (!curve) {
+                                float gs1 = 0.f;
+                                V3 gs3 = v3s(0.f), gang = v3s(0.f), gtr = v3s(0.f);
+                                Frame F;
+                                F.trans = ld3(rec + R_TR);
+                                F.angle = ld3(rec + R_ANG);
+                                F.scale = ld3(rec + R_SC);
+#pragma unroll
+                                for (int ii = 0; ii < 3; ii++)
+#pragma unroll
+                                    for (int jj = 0; jj < 3; jj++) F.R.r[ii][jj] = rec[R_ROT + 3 * ii + jj];
+                                if (A.exact && A.isr && (flags & (F_SS | F_MS))) {
+                                    const float gsc = ga * dalpha_dscale(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0);
+                                    if (flags & F_SS) gs1 += gsc * q.sc / F.scale.x;
+                                    else radius_factor_adjoint(base, q.p, prm, F.scale, gsc * q.sc / q.fac, gpos, gs3, G);
+                                }
+                                xform_adjoint(gpos, q.p, x, F, flags, gs1, gs3, gang, gtr);
+                                if (flags & F_SS) G[2] += gs1;
+                                else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
+                                G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
+                                G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
+                            }
+                        }
+                    }
+                    // segmented sum over the lanes of each stroke through an LDS transpose
+#pragma unroll
+                    for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
+                    wave_sync();
+                    const int nl = min(64, wp - p0);
+                    const int cfirst = __builtin_amdgcn_readfirstlane(c);
+                    const int clast = __builtin_amdgcn_readlane(c, nl - 1);
+                    const int items = (clast - cfirst + 1) * NG;
+                    for (int it = lane; it < items; it += 64) {
+                        const int jseg = it / NG;
+                        const int g = it - jseg * NG;
+                        const int cc = cfirst + jseg;
+                        const int rel = w.coff[cc] - wbase - p0;
+                        const int lo = max(rel, 0);
+                        const int hi = min(rel + w.ccnt[cc], nl);
+                        float sum = 0.f;
+                        for (int l = lo; l < hi; l++) sum += w.scr[l * (NG + 1) + g];
+                        if (sum != 0.f) {
+                            const int k = w.ck[cc];
+                            const int col = slot_column(g, A.table + (int64_t)k * REC, A.Pmax, A.Q);
+                            if (col >= 0) atomicAdd(&dst[(int64_t)k * PG + col], sum);
+                        }
+                    }
+                    wave_sync();
+                }
+                c0 = c1;
             }
         }
     }
@@ -765,10 +952,25 @@ __global__ void render_reduce_kernel(const float* __restrict__ partials, int npa
     else gdens[k] += s;
 }
 
+__global__ void fill_zero_kernel(float* __restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
 static int g_num_cu = 0;
 static int g_max_lds = 0;
 
+typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
 typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
+
+static FwdKernel fwd_kernel_ptr(int ns, int cid) {
+    static const FwdKernel tab[4][4] = {
+        {render_fwd_kernel<1, 0>, render_fwd_kernel<1, 1>, render_fwd_kernel<1, 2>, render_fwd_kernel<1, 3>},
+        {render_fwd_kernel<2, 0>, render_fwd_kernel<2, 1>, render_fwd_kernel<2, 2>, render_fwd_kernel<2, 3>},
+        {render_fwd_kernel<3, 0>, render_fwd_kernel<3, 1>, render_fwd_kernel<3, 2>, render_fwd_kernel<3, 3>},
+        {render_fwd_kernel<4, 0>, render_fwd_kernel<4, 1>, render_fwd_kernel<4, 2>, render_fwd_kernel<4, 3>}};
+    return tab[ns - 1][cid];
+}
 
 static BwdKernel bwd_kernel_ptr(int ns, int cid) {
     static const BwdKernel tab[4][4] = {
@@ -779,12 +981,39 @@ static BwdKernel bwd_kernel_ptr(int ns, int cid) {
     return tab[ns - 1][cid];
 }
 
-// upper bound on persistent backward blocks: 8 per CU, never more than one per 4 rays
+static int fwd_lds_bytes(int ns, int cid) { return FWAVES * wave_mem_floats(ns, cid == 1 ? 4 : 1, 0) * 4; }
+
+static int bwd_wave_floats(int ns, int cid) { return wave_mem_floats(ns, 5, 15 + num_color_params(cid)); }
+
+// upper bound on persistent backward blocks: 8 per CU, never more than one per ray
 static int64_t max_persistent_blocks(int64_t n_rays) {
     int64_t nb = (int64_t)g_num_cu * 8;
-    int64_t need = (n_rays + RWAVES - 1) / RWAVES;
-    if (nb > need) nb = need;
+    if (nb > n_rays) nb = n_rays;
     return nb < 1 ? 1 : nb;
+}
+
+// Backward launch shape: wavefronts per block and whether the [S, PG] accumulators fit in LDS.
+struct BwdShape {
+    int waves, use_lds;
+    int64_t lds_bytes;
+};
+
+static BwdShape bwd_shape(int ns, int cid, int S, int PG) {
+    BwdShape b;
+    const int64_t wave_bytes = (int64_t)bwd_wave_floats(ns, cid) * 4;
+    const int64_t acc_bytes = (int64_t)S * PG * 4;
+    for (int wv = BWAVES; wv >= 2; wv >>= 1) {
+        if (acc_bytes + wv * wave_bytes <= g_max_lds) {
+            b.waves = wv;
+            b.use_lds = 1;
+            b.lds_bytes = acc_bytes + wv * wave_bytes;
+            return b;
+        }
+    }
+    b.waves = BWAVES;  // accumulators in global memory
+    b.use_lds = 0;
+    b.lds_bytes = BWAVES * wave_bytes;
+    return b;
 }
 
 static int device_info() {
@@ -854,16 +1083,10 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                              n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
                              use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps, 0);
-    dim3 grid((unsigned)((n_rays + RWAVES - 1) / RWAVES)), block(64 * RWAVES);
-    hipStream_t s = (hipStream_t)stream;
-    int ns = (n_samples + 63) / 64;
-    typedef void (*Fwd)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
-    static const Fwd tab[4][4] = {
-        {render_fwd_kernel<1, 0>, render_fwd_kernel<1, 1>, render_fwd_kernel<1, 2>, render_fwd_kernel<1, 3>},
-        {render_fwd_kernel<2, 0>, render_fwd_kernel<2, 1>, render_fwd_kernel<2, 2>, render_fwd_kernel<2, 3>},
-        {render_fwd_kernel<3, 0>, render_fwd_kernel<3, 1>, render_fwd_kernel<3, 2>, render_fwd_kernel<3, 3>},
-        {render_fwd_kernel<4, 0>, render_fwd_kernel<4, 1>, render_fwd_kernel<4, 2>, render_fwd_kernel<4, 3>}};
-    hipLaunchKernelGGL(tab[ns - 1][color_id], grid, block, 0, s, A, rgb_out, acc_out, depth_out, state_out, counters);
+    dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
+    const int ns = (n_samples + 63) / 64;
+    hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
+                       A, rgb_out, acc_out, depth_out, state_out, counters);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;
 }
@@ -872,9 +1095,13 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
 N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_strokes, int shape_width,
                                                           int color_width) {
     if (device_info()) return -1;
-    int64_t PG = shape_width + color_width + 1;
-    int64_t lds = (int64_t)n_strokes * PG * 4;
-    if (lds > g_max_lds) return (int64_t)n_strokes * PG;  // global-atomic path
+    const int64_t PG = shape_width + color_width + 1;
+    int cid = 0;
+    for (int c = 0; c < 4; c++)
+        if (num_color_params(c) == color_width) cid = c;
+    bool lds = false;
+    for (int ns = 1; ns <= 4; ns++) lds = lds || bwd_shape(ns, cid, n_strokes, (int)PG).use_lds;
+    if (!lds) return (int64_t)n_strokes * PG;  // global-atomic path
     return max_persistent_blocks(n_rays) * n_strokes * PG;
 }
 
@@ -896,41 +1123,42 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
                              n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
                              use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
                              adjoint_exact);
-    const int64_t PG = shape_width + color_width + 1;
-    const int64_t lds = (int64_t)n_strokes * PG * 4;
-    const int use_lds = lds <= g_max_lds;
-    const int ns_ = (n_samples + 63) / 64;
+    const int PG = shape_width + color_width + 1;
+    const int ns = (n_samples + 63) / 64;
+    const BwdShape bs = bwd_shape(ns, color_id, n_strokes, PG);
+    N3DS_REQUIRE(bs.lds_bytes <= g_max_lds, "backward needs %lld bytes of LDS per block", (long long)bs.lds_bytes);
+    BwdKernel kern = bwd_kernel_ptr(ns, color_id);
+    if (bs.lds_bytes > 65536)
+        N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bs.lds_bytes));
+    // as many persistent blocks as fit on the device at once (registers and LDS)
     int64_t nb = max_persistent_blocks(n_rays);
     {
-        // as many persistent blocks as fit on the device at once (registers and LDS)
+        const int64_t need = (n_rays + bs.waves - 1) / bs.waves;
+        if (nb > need) nb = need;
         int per_cu = 0;
-        BwdKernel kq = bwd_kernel_ptr(ns_, color_id);
-        if (use_lds && lds > 65536)
-            N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kq, 64 * RWAVES,
-                                                                    use_lds ? (size_t)lds : 0));
+        N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * bs.waves,
+                                                                    (size_t)bs.lds_bytes));
         if (per_cu < 1) per_cu = 1;
-        int64_t fit = (int64_t)g_num_cu * per_cu;
+        const int64_t fit = (int64_t)g_num_cu * per_cu;
         if (fit < nb) nb = fit;
     }
-    const int64_t ws_need = use_lds ? nb * n_strokes * PG : (int64_t)n_strokes * PG;
+    const int64_t tot = (int64_t)n_strokes * PG;
+    const int64_t ws_need = bs.use_lds ? nb * tot : tot;
     N3DS_REQUIRE(workspace_floats >= ws_need, "workspace has %lld floats, need %lld", (long long)workspace_floats,
                  (long long)ws_need);
     hipStream_t s = (hipStream_t)stream;
-    if (!use_lds) N3DS_HIP_CHECK(hipMemsetAsync(workspace, 0, sizeof(float) * ws_need, s));
-    dim3 grid((unsigned)nb), block(64 * RWAVES);
-    size_t shm = use_lds ? (size_t)lds : 0;
-    float* parts = use_lds ? workspace : nullptr;
-    float* direct = use_lds ? nullptr : workspace;
-    int ns = (n_samples + 63) / 64;
-    BwdKernel kern = bwd_kernel_ptr(ns, color_id);
-    if (shm > 65536) N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern,
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(kern, grid, block, shm, s, A, grad_rgb, grad_acc, grad_depth, state, parts, use_lds, direct);
+    if (!bs.use_lds) {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace, tot);
+        N3DS_LAUNCH_CHECK();
+    }
+    float* parts = bs.use_lds ? workspace : nullptr;
+    float* direct = bs.use_lds ? nullptr : workspace;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * bs.waves), (size_t)bs.lds_bytes, s, A, grad_rgb, grad_acc,
+                       grad_depth, state, parts, bs.use_lds, direct);
     N3DS_LAUNCH_CHECK();
-    int64_t tot = (int64_t)n_strokes * PG;
     hipLaunchKernelGGL(render_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace,
-                       use_lds ? (int)nb : 1, n_strokes, shape_width, color_width, grad_shape, grad_color,
+                       bs.use_lds ? (int)nb : 1, n_strokes, shape_width, color_width, grad_shape, grad_color,
                        grad_density);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;

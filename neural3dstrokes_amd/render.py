@@ -26,9 +26,10 @@ N_BASE = 12
 
 
 def prepare_table(stroke_ids, shape_params):
-    """Per-stroke record table [S, 64] consumed by the render kernels."""
+    """Per-stroke record table consumed by the render kernels (size from the library)."""
     S, Pmax = shape_params.shape
-    table = torch.empty(S, 64, dtype=torch.float32, device=shape_params.device)
+    nt = int(_lib.lib().n3ds_render_table_floats(S))
+    table = torch.empty(max(nt, 1), dtype=torch.float32, device=shape_params.device)
     status = torch.zeros(1, dtype=torch.int32, device=shape_params.device)
     _lib.check(_lib.lib().n3ds_render_prepare(_lib.ptr(table), _lib.ptr(stroke_ids), _lib.ptr(shape_params), S, Pmax,
                                               _lib.ptr(status), _lib.stream()))

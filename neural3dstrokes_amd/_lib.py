@@ -11,7 +11,7 @@ import os
 
 import torch
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libn3ds.so')
+LIB_PATH = os.environ.get('N3DS_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libn3ds.so')
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64

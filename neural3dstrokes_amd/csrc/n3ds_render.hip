@@ -38,8 +38,8 @@
 
 namespace n3ds {
 
-constexpr int REC = 64;      // floats per stroke record
-constexpr int NPIECE = 5;    // curve pieces bounded for culling
+constexpr int REC = 128;     // floats per stroke record
+constexpr int NPIECE = 16;   // curve pieces bounded for culling
 constexpr int FWAVES = 4;    // wavefronts per forward block
 constexpr int BWAVES = 8;    // wavefronts per backward block (at most)
 constexpr int PBUF = 256;    // (sample, stroke) pairs staged per window (>= samples per ray)
@@ -57,9 +57,10 @@ enum : int {
     R_E1 = 32,      // 3: margin multiplier
     R_CP = 8,       // 14: curve params (curves reuse the solid slots)
     R_BOX = 22,     // NPIECE * 6: piece boxes
-    R_HULL = 52,    // 6: whole-curve box
-    R_RMAX = 58,    // max tube radius
+    R_HULL = 118,   // 6: whole-curve box
+    R_RMAX = 124,   // max tube radius
 };
+static_assert(R_BOX + 6 * NPIECE <= R_HULL && R_RMAX < REC, "stroke record layout");
 
 struct RenderArgs {
     const float* origins;
@@ -208,24 +209,32 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
     t1 = fminf(t1, fmaxf(a, b));
 }
 
-__device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o, V3 d, float m, int isr, float tlo,
-                                            float thi, float& ta, float& tb) {
+// slab test with a precomputed inverse direction component
+__device__ __forceinline__ void slab_inv(float o, float inv, float lo, float hi, float& t0, float& t1) {
+    float a = (lo - o) * inv, b = (hi - o) * inv;
+    t0 = fmaxf(t0, fminf(a, b));
+    t1 = fminf(t1, fmaxf(a, b));
+}
+
+// di: componentwise 1 / d (curves are tested in world space, so it is per ray)
+__device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o, V3 d, V3 di, float m, int isr,
+                                            float tlo, float thi, float& ta, float& tb) {
     float t0 = tlo, t1 = thi;
     if (ibits(rec[R_CURVE])) {
         float e = rec[R_RMAX] + m;
         float h0 = tlo, h1 = thi;
-        slab(o.x, d.x, rec[R_HULL + 0] - e, rec[R_HULL + 3] + e, h0, h1);
-        slab(o.y, d.y, rec[R_HULL + 1] - e, rec[R_HULL + 4] + e, h0, h1);
-        slab(o.z, d.z, rec[R_HULL + 2] - e, rec[R_HULL + 5] + e, h0, h1);
+        slab_inv(o.x, di.x, rec[R_HULL + 0] - e, rec[R_HULL + 3] + e, h0, h1);
+        slab_inv(o.y, di.y, rec[R_HULL + 1] - e, rec[R_HULL + 4] + e, h0, h1);
+        slab_inv(o.z, di.z, rec[R_HULL + 2] - e, rec[R_HULL + 5] + e, h0, h1);
         if (!(h0 <= h1)) return false;
         float u0 = 1e30f, u1 = -1e30f;
-#pragma unroll
+#pragma unroll 4
         for (int j = 0; j < NPIECE; j++) {
             const float* bx = rec + R_BOX + 6 * j;
-            float p0 = tlo, p1 = thi;
-            slab(o.x, d.x, bx[0] - e, bx[3] + e, p0, p1);
-            slab(o.y, d.y, bx[1] - e, bx[4] + e, p0, p1);
-            slab(o.z, d.z, bx[2] - e, bx[5] + e, p0, p1);
+            float p0 = h0, p1 = h1;
+            slab_inv(o.x, di.x, bx[0] - e, bx[3] + e, p0, p1);
+            slab_inv(o.y, di.y, bx[1] - e, bx[4] + e, p0, p1);
+            slab_inv(o.z, di.z, bx[2] - e, bx[5] + e, p0, p1);
             if (p0 <= p1) { u0 = fminf(u0, p0); u1 = fmaxf(u1, p1); }
         }
         t0 = u0;
@@ -299,6 +308,7 @@ __device__ __forceinline__ V3 stroke_color_uniform(const float* __restrict__ cp,
 // ---------------------------------------------------------------------------
 struct RayConst {
     V3 o, d, vd, oc, dc;  // oc, dc: origin / direction in the (warped) stroke coordinates
+    V3 di;                // 1 / dc componentwise
     float radii, tlo, thi, m;
 };
 
@@ -319,6 +329,7 @@ __device__ __forceinline__ RayConst load_ray(const RenderArgs& A, int64_t ray) {
         c.oc = v3(c.o.x / A.coord_div, c.o.y / A.coord_div, c.o.z / A.coord_div);
         c.dc = v3(c.d.x / A.coord_div, c.d.y / A.coord_div, c.d.z / A.coord_div);
     }
+    c.di = v3(1.f / c.dc.x, 1.f / c.dc.y, 1.f / c.dc.z);
     return c;
 }
 
@@ -429,7 +440,7 @@ __device__ __forceinline__ int collect_hits(const RenderArgs& A, const RayConst&
                                             int base0, int lane) {
     const int k = A.S - 1 - (base0 + lane);
     float ta = 0.f, tb = -1.f;
-    bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.m, A.isr, rc.tlo, rc.thi, ta, tb);
+    bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi, ta, tb);
     int i0 = 0, cnt = 0;
     if (hit) {
         if (sorted) {

@@ -52,7 +52,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(handle, name), name
     lib = _lib.lib()
     assert lib.n3ds_version() == 1
-    assert lib.n3ds_render_table_floats(10) == 640
+    assert lib.n3ds_render_table_floats(10) == 1280  # 128 floats per stroke record
     assert lib.n3ds_last_error() == b''
 
 

@@ -153,14 +153,22 @@ int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* sh
  * cone != 0: sample radius = radii * t; coord_div divides sample coordinates
  * (models.py:663 warp); bg_color is a host pointer to 3 floats; cull_eps is the
  * alpha below which the Laplace transfer's tail is culled (linear transfer:
- * culling is exact). */
+ * culling is exact).
+ * Optional activations kept for the backward (hit_dir == NULL: none, the
+ * backward recomputes everything): hit_dir int32 [R, ceil(S/64), 3],
+ * hit_buf int32 [R, cap_hits, 4], pair_buf float [R, npf, cap_pairs] with
+ * npf = 4 for color_id 1 (gradient_rgb) else 1.  Per ray, the culled hits
+ * and the alpha (and colour) of every evaluated (sample, stroke) pair are
+ * kept chunk by chunk (64 strokes) until the ray's capacity is used up; the
+ * backward recomputes the chunks that did not fit. */
 int n3ds_render_forward(const float* origins, const float* directions, const float* viewdirs, const float* radii,
                         const float* tdist, const float* table, const float* color_params,
                         const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
                         int shape_width, int color_width, int color_id, float sdf_delta,
                         int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
                         const float* bg_color, float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
-                        float* state_out, unsigned long long* counters, void* stream);
+                        float* state_out, unsigned long long* counters, int* hit_dir, int* hit_buf,
+                        float* pair_buf, int cap_hits, int cap_pairs, void* stream);
 
 /* Floats of device workspace n3ds_render_backward needs (queries the current
  * device; -1 on a HIP error). */
@@ -168,15 +176,18 @@ int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_strokes, int
 
 /* Backward: accumulates dL/d(shape_params) [S,shape_width], dL/d(color_params)
  * [S,color_width] and dL/d(density_params) [S] for upstream grad_rgb [R,3],
- * grad_acc [R] (or NULL) and grad_depth [R] (or NULL), recomputing the SDFs
- * from the forward's `state`.  adjoint_exact as for n3ds_stroke_backward. */
+ * grad_acc [R] (or NULL) and grad_depth [R] (or NULL), from the forward's
+ * `state` and, when given, the hits / pair values it kept (same buffers and
+ * capacities as passed to n3ds_render_forward); the per-pair SDF gradients
+ * are always recomputed.  adjoint_exact as for n3ds_stroke_backward. */
 int n3ds_render_backward(const float* origins, const float* directions, const float* viewdirs, const float* radii,
                          const float* tdist, const float* table, const float* color_params,
                          const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
                          int shape_width, int color_width, int color_id, float sdf_delta,
                          int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
                          const float* bg_color, float cull_eps, int adjoint_exact, const float* grad_rgb,
-                         const float* grad_acc, const float* grad_depth, const float* state, float* workspace,
+                         const float* grad_acc, const float* grad_depth, const float* state, const int* hit_dir,
+                         const int* hit_buf, const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
                          int64_t workspace_floats, float* grad_shape, float* grad_color, float* grad_density,
                          void* stream);
 

@@ -78,6 +78,14 @@ struct RenderArgs {
     float coord_div;
     float bg0, bg1, bg2;
     float tail;  // culling margin multiplier
+    // hits and per-pair values the forward keeps for the adjoint (hdir == nullptr: none).
+    // hdir [R, nch, 3]: per 64-stroke chunk (hit offset or -1 if not kept, pair offset, hits);
+    // hbuf [R, cap_h, 4]: stroke, first sample, samples, pair offset within the chunk;
+    // pbuf [R, npf, cap_p]: alpha (+ gradient_rgb colour) planes.
+    int* hdir;
+    int* hbuf;
+    float* pbuf;
+    int cap_h, cap_p, nch;
 };
 
 __device__ __forceinline__ int ibits(float f) { return __float_as_int(f); }
@@ -494,8 +502,38 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
 #pragma unroll
     for (int s = 0; s < NS; s++) { T[s] = 1.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
     unsigned long long my_pos = 0, my_ev = 0, my_vis = 0;  // lane b counts stroke type b (counting pass)
+    // cursors into this ray's region of the kept hits / pair values; once a chunk
+    // does not fit, it and every later chunk are left to the adjoint to recompute
+    int hoff = 0, poff = 0;
+    bool full = false;
+    const int64_t rh = ray * (int64_t)A.cap_h;
+    const int64_t rp = ray * (int64_t)A.cap_p * NPF;
     for (int base0 = 0; base0 < A.S; base0 += 64) {
         const int nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
+        bool keep = false;
+        const int chunk_poff = poff;
+        if (A.hdir) {
+            const int tot = nc ? w.coff[nc - 1] + w.ccnt[nc - 1] : 0;
+            keep = !full && hoff + nc <= A.cap_h && poff + tot <= A.cap_p;
+            full = !keep;
+            if (lane == 0) {
+                int* e = A.hdir + (ray * A.nch + base0 / 64) * 3;
+                e[0] = keep ? hoff : -1;
+                e[1] = poff;
+                e[2] = nc;
+            }
+            if (keep && lane < nc) {
+                int* h = A.hbuf + (rh + hoff + lane) * 4;
+                h[0] = w.ck[lane];
+                h[1] = w.ci0[lane];
+                h[2] = w.ccnt[lane];
+                h[3] = w.coff[lane];
+            }
+            if (keep) {
+                hoff += nc;
+                poff += tot;
+            }
+        }
         for (int c0 = 0; c0 < nc;) {
             const int wbase = w.coff[c0];
             const int c1 = window_end(w, c0, nc, wbase, lane);
@@ -513,11 +551,18 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
                     const float* rec = A.table + (int64_t)k * REC;
                     const Pair q = eval_pair(rec, ibits(rec[R_BASE]), ibits(rec[R_CURVE]) != 0, x, r, A);
                     w.pb[p] = q.alpha;
+                    if (keep) A.pbuf[rp + chunk_poff + wbase + p] = q.alpha;
                     if (CID == 1) {
                         const V3 col = color_eval_t<1>(q.p, rc.vd, A.cparams + (int64_t)k * A.Q);
                         w.pb[PBUF + p] = col.x;
                         w.pb[2 * PBUF + p] = col.y;
                         w.pb[3 * PBUF + p] = col.z;
+                        if (keep) {
+                            float* kp = A.pbuf + rp + chunk_poff + wbase + p;
+                            kp[A.cap_p] = col.x;
+                            kp[2 * (int64_t)A.cap_p] = col.y;
+                            kp[3 * (int64_t)A.cap_p] = col.z;
+                        }
                     }
                 }
             }
@@ -777,14 +822,46 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
             C2[s] = v3s(0.f);
         }
         // ---- strokes in compositing order
+        const int64_t rh = ray * (int64_t)A.cap_h;
+        const int64_t rp = ray * (int64_t)A.cap_p * (CID == 1 ? 4 : 1);
         for (int base0 = 0; base0 < A.S; base0 += 64) {
-            const int nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
+            // hits of this chunk: kept by the forward, or culled again
+            int nc = 0, kpoff = -1;
+            if (A.hdir) {
+                const int* e = A.hdir + (ray * A.nch + base0 / 64) * 3;
+                const int hoff = __builtin_amdgcn_readfirstlane(e[0]);
+                if (hoff >= 0) {
+                    kpoff = __builtin_amdgcn_readfirstlane(e[1]);
+                    nc = __builtin_amdgcn_readfirstlane(e[2]);
+                    if (lane < nc) {
+                        const int* h = A.hbuf + (rh + hoff + lane) * 4;
+                        w.ck[lane] = h[0];
+                        w.ci0[lane] = h[1];
+                        w.ccnt[lane] = h[2];
+                        w.coff[lane] = h[3];
+                    }
+                    wave_sync();
+                }
+            }
+            if (kpoff < 0) nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
             for (int c0 = 0; c0 < nc;) {
                 const int wbase = w.coff[c0];
                 const int c1 = window_end(w, c0, nc, wbase, lane);
                 const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
-                // (1) alpha (and gradient_rgb colour) per pair
-                for (int p0 = 0; p0 < wp; p0 += 64) {
+                // (1) alpha (and gradient_rgb colour) per pair: kept values or recomputed
+                for (int p0 = 0; kpoff >= 0 && p0 < wp; p0 += 64) {
+                    const int p = p0 + lane;
+                    if (p < wp) {
+                        const float* kp = A.pbuf + rp + kpoff + wbase + p;
+                        w.pb[p] = kp[0];
+                        if (CID == 1) {
+                            w.pb[PBUF + p] = kp[A.cap_p];
+                            w.pb[2 * PBUF + p] = kp[2 * (int64_t)A.cap_p];
+                            w.pb[3 * PBUF + p] = kp[3 * (int64_t)A.cap_p];
+                        }
+                    }
+                }
+                for (int p0 = 0; kpoff < 0 && p0 < wp; p0 += 64) {
                     const int p = p0 + lane;
                     if (p < wp) {
                         const int c = find_hit(w, c0, c1, wbase, p);
@@ -1065,7 +1142,24 @@ static RenderArgs make_args(const float* origins, const float* dirs, const float
     } else {
         A.tail = 1.001f;
     }
+    A.hdir = nullptr;
+    A.hbuf = nullptr;
+    A.pbuf = nullptr;
+    A.cap_h = A.cap_p = 0;
+    A.nch = (S + 63) / 64;
     return A;
+}
+
+static int set_keep(RenderArgs& A, int* hit_dir, int* hit_buf, float* pair_buf, int cap_hits, int cap_pairs) {
+    if (!hit_dir) return N3DS_OK;
+    N3DS_REQUIRE(hit_buf && pair_buf && cap_hits > 0 && cap_pairs > 0,
+                 "kept hits need hit_buf, pair_buf and positive capacities");
+    A.hdir = hit_dir;
+    A.hbuf = hit_buf;
+    A.pbuf = pair_buf;
+    A.cap_h = cap_hits;
+    A.cap_p = cap_pairs;
+    return N3DS_OK;
 }
 
 N3DS_EXPORT int64_t n3ds_render_table_floats(int n_strokes) { return (int64_t)n_strokes * REC; }
@@ -1087,13 +1181,16 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
                                     float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
                                     float coord_div, const float* bg_color, float cull_eps, float* rgb_out,
                                     float* acc_out, float* depth_out, float* state_out,
-                                    unsigned long long* counters, void* stream) {
+                                    unsigned long long* counters, int* hit_dir, int* hit_buf, float* pair_buf,
+                                    int cap_hits, int cap_pairs, void* stream) {
     int st = render_check(n_rays, n_samples, n_strokes, shape_width, color_width, color_id);
     if (st) return st;
     if (n_rays == 0) return N3DS_OK;
     RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                              n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
                              use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps, 0);
+    st = set_keep(A, hit_dir, hit_buf, pair_buf, cap_hits, cap_pairs);
+    if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
     hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
@@ -1123,8 +1220,10 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
                                      float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
                                      float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
                                      const float* grad_rgb, const float* grad_acc, const float* grad_depth,
-                                     const float* state, float* workspace, int64_t workspace_floats,
-                                     float* grad_shape, float* grad_color, float* grad_density, void* stream) {
+                                     const float* state, const int* hit_dir, const int* hit_buf,
+                                     const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
+                                     int64_t workspace_floats, float* grad_shape, float* grad_color,
+                                     float* grad_density, void* stream) {
     int st = render_check(n_rays, n_samples, n_strokes, shape_width, color_width, color_id);
     if (st) return st;
     st = device_info();
@@ -1134,6 +1233,9 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
                              n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
                              use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
                              adjoint_exact);
+    st = set_keep(A, const_cast<int*>(hit_dir), const_cast<int*>(hit_buf), const_cast<float*>(pair_buf), cap_hits,
+                  cap_pairs);
+    if (st) return st;
     const int PG = shape_width + color_width + 1;
     const int ns = (n_samples + 63) / 64;
     const BwdShape bs = bwd_shape(ns, color_id, n_strokes, PG);

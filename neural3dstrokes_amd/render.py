@@ -36,12 +36,43 @@ def prepare_table(stroke_ids, shape_params):
     return table, status
 
 
+# Kept activations ("store"): per ray, the forward keeps its culled hits and the
+# alpha of each evaluated (sample, stroke) pair so the backward neither culls nor
+# evaluates alpha again; a ray that outgrows its capacity falls back to
+# recomputation chunk by chunk.  Capacities per ray: hits min(S, 2n), pairs 8n.
+STORE_BUDGET_BYTES = 8 << 30
+
+
+def keep_capacities(R, n, S, color_id):
+    cap_h = max(1, min(S, 2 * n))
+    cap_p = 8 * n
+    npf = 4 if color_id == 1 else 1
+    nch = (S + 63) // 64
+    nbytes = R * (cap_h * 16 + cap_p * npf * 4 + nch * 12)
+    return cap_h, cap_p, npf, nch, nbytes
+
+
+def _keep_buffers(store, R, n, S, color_id, dev):
+    if store is False:
+        return None
+    if isinstance(store, tuple):
+        cap_h, cap_p = store
+        npf, nch = (4 if color_id == 1 else 1), (S + 63) // 64
+    else:
+        cap_h, cap_p, npf, nch, nbytes = keep_capacities(R, n, S, color_id)
+        if store == 'auto' and nbytes > STORE_BUDGET_BYTES:
+            return None
+    return (torch.empty(R * nch * 3, dtype=torch.int32, device=dev),
+            torch.empty(R * cap_h * 4, dtype=torch.int32, device=dev),
+            torch.empty(R * npf * cap_p, dtype=torch.float32, device=dev), cap_h, cap_p)
+
+
 class _render_fn(Function):
 
     @staticmethod
     def forward(ctx, origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                 density_params, cfg):
-        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters) = cfg
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store) = cfg
         R, n1 = tdist.shape
         S, Pmax = shape_params.shape
         for name, t in (('origins', origins), ('directions', directions), ('viewdirs', viewdirs), ('radii', radii),
@@ -57,14 +88,21 @@ class _render_fn(Function):
         acc = torch.empty(R, dtype=torch.float32, device=dev)
         depth = torch.empty(R, dtype=torch.float32, device=dev)
         state = torch.empty(5, R * (n1 - 1), dtype=torch.float32, device=dev)
+        need_grad = any(ctx.needs_input_grad[6:9])
+        keep = _keep_buffers(store, R, n1 - 1, S, color_id, dev) if need_grad else None
+        hd, hb, pb, cap_h, cap_p = keep if keep is not None else (None, None, None, 0, 0)
         _lib.check(_lib.lib().n3ds_render_forward(
             _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
             _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), R, n1 - 1, S, Pmax,
             color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
             _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), float(cull_eps), _lib.ptr(rgb), _lib.ptr(acc),
-            _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.stream()))
+            _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.ptr(hd), _lib.ptr(hb), _lib.ptr(pb), cap_h,
+            cap_p, _lib.stream()))
+        empty = torch.empty(0, device=dev)
         ctx.save_for_backward(origins, directions, viewdirs, radii, tdist, table, shape_params, color_params,
-                              density_params, state)
+                              density_params, state, hd if hd is not None else empty,
+                              hb if hb is not None else empty, pb if pb is not None else empty)
+        ctx.caps = (cap_h, cap_p)
         ctx.cfg = cfg
         ctx.bgv = bgv
         return rgb, acc, depth
@@ -72,8 +110,9 @@ class _render_fn(Function):
     @staticmethod
     def backward(ctx, g_rgb, g_acc, g_depth):
         (origins, directions, viewdirs, radii, tdist, table, shape_params, color_params, density_params,
-         state) = ctx.saved_tensors
-        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters) = ctx.cfg
+         state, hd, hb, pb) = ctx.saved_tensors
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store) = ctx.cfg
+        cap_h, cap_p = ctx.caps
         R, n1 = tdist.shape
         S, Pmax = shape_params.shape
         Q = color_params.shape[1]
@@ -93,8 +132,9 @@ class _render_fn(Function):
             _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), R, n1 - 1, S, Pmax, Q, color_id,
             float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
             _lib.ctypes.cast(ctx.bgv, _lib.ctypes.c_void_p), float(cull_eps), 1 if adjoint == 'exact' else 0,
-            _lib.ptr(g_rgb), _lib.ptr(g_acc), _lib.ptr(g_depth), _lib.ptr(state), _lib.ptr(ws), int(ws.numel()),
-            _lib.ptr(gs), _lib.ptr(gc), _lib.ptr(gd), _lib.stream()))
+            _lib.ptr(g_rgb), _lib.ptr(g_acc), _lib.ptr(g_depth), _lib.ptr(state), _lib.ptr(hd), _lib.ptr(hb),
+            _lib.ptr(pb), cap_h, cap_p, _lib.ptr(ws), int(ws.numel()), _lib.ptr(gs), _lib.ptr(gc), _lib.ptr(gd),
+            _lib.stream()))
         return (None, None, None, None, None, None, gs if ctx.needs_input_grad[6] else None,
                 gc if ctx.needs_input_grad[7] else None, gd if ctx.needs_input_grad[8] else None, None)
 
@@ -102,7 +142,7 @@ class _render_fn(Function):
 def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                    density_params, color_id=0, sdf_delta=1.0, use_laplace_transform=False, inv_scale_radius=False,
                    cone=False, coord_div=1.0, bg_color=(1.0, 1.0, 1.0), cull_eps=1e-7, adjoint=None,
-                   counters=None):
+                   counters=None, store='auto'):
     """Render rays through a stroke field; returns (rgb [R,3], acc [R], depth [R]).
 
     origins/directions/viewdirs [R,3], radii [R] (sample radius = radii·t if ``cone``
@@ -111,6 +151,13 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
     density_params [S].  Sample coordinates are ``(o + d·t) / coord_div`` (the
     ``_warp_coords`` bound of ``models.py:19-33``).  Gradients flow to
     shape_params, color_params and density_params.
+
+    ``store``: what the forward keeps for the backward.  ``False``: only the
+    per-sample composite state (20 B per ray-sample; the backward recomputes
+    culling and alpha); ``True``: also the culled hits and per-pair alpha
+    (``keep_capacities``); ``'auto'``: ``True`` when that fits
+    ``STORE_BUDGET_BYTES``; a ``(cap_hits, cap_pairs)`` tuple sets the per-ray
+    capacities.  Results are identical in every mode.
     """
     if color_params.shape[1] != COLOR_WIDTH[color_id]:
         raise RuntimeError(f'color_params has width {color_params.shape[1]}, color_id {color_id} needs '
@@ -123,7 +170,7 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
                 raise RuntimeError(f'invalid stroke id {i}')
         stroke_ids = stroke_ids.to(device=shape_params.device, dtype=torch.int32)
     cfg = (int(color_id), float(sdf_delta), bool(use_laplace_transform), bool(inv_scale_radius), bool(cone),
-           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters)
+           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters, store)
     f = lambda t: t.contiguous().float()
     return _render_fn.apply(f(origins), f(directions), f(viewdirs), f(radii), f(tdist), stroke_ids.contiguous(),
                             f(shape_params), f(color_params), f(density_params), cfg)

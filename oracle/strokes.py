@@ -142,8 +142,14 @@ def _lerp(a, b, t):
 
 
 def _sigmoid(v):
-    """common.h:41-45 ``1 / (1 + exp(-v))``."""
-    return 1.0 / (1.0 + torch.exp(-v))
+    """common.h:41-45 ``1 / (1 + exp(-v))``.
+
+    The value is that formula; its autograd derivative is sigma (1 - sigma)
+    (torch.sigmoid's), which stays finite where exp(-v) overflows and the
+    formula's own derivative would be inf / inf.
+    """
+    s = torch.sigmoid(v)
+    return s + (1.0 / (1.0 + torch.exp(-v)) - s).detach()
 
 
 # ----------------------------------------------------------------------------

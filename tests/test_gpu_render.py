@@ -57,6 +57,28 @@ def test_render_exact(name, n_rays, n_samples, n_strokes, dev):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('store', [False, True, (6, 40)])
+@pytest.mark.parametrize('color_id', [0, 1])
+def test_render_kept_activations(store, color_id, dev):
+    """Backward from kept hits / alpha, from recomputation, and mixed (tiny per-ray
+    capacities: early chunks kept, later ones recomputed) give the same gradients."""
+    sc = _scene('cfg1', 48, 128, 150, seed=3)
+    if color_id == 1:
+        g = torch.Generator().manual_seed(5)
+        sc['color_params'] = torch.cat([torch.rand(150, 6, generator=g) - 0.5, torch.rand(150, 6, generator=g)], -1)
+    (rgb, acc, _), (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev, store=store, color_id=color_id)
+    out, rs, rc, rd = ofield.render_field_grads(g_rgb.to(D64), g_acc.to(D64), *_oracle_args(sc), color_id=color_id)
+    assert rel_err(rgb, out['rgb']) < 1e-5
+    assert rel_err(acc, out['acc']) < 1e-5
+    assert rel_err(gs, rs) < 1e-4, rel_err(gs, rs)
+    assert rel_err(gc, rc) < 1e-4
+    assert rel_err(gd, rd) < 1e-4
+    if store is not False:
+        (_, _, _), (gs2, gc2, gd2), _ = _run(sc, dev, store=False, color_id=color_id)
+        assert rel_err(gs, gs2) < 1e-5 and rel_err(gc, gc2) < 1e-5 and rel_err(gd, gd2) < 1e-5
+
+
+@pytest.mark.gpu
 def test_render_reference_adjoint(dev):
     sc = _scene('cfg1', 32, 128, 80, seed=3)
     _, (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev, adjoint='reference', seed=3)

@@ -365,19 +365,26 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     return v;
 }
 
+// Hits are gathered over consecutive 64-stroke chunks (in compositing order)
+// until HMAX - 64 hits or FLUSH_PAIRS pairs are pending, so that the
+// one-pair-per-lane batches are full even though a chunk typically covers
+// only a few (sample, stroke) pairs of a ray.
+constexpr int HMAX = 128;         // pending hits per wavefront
+constexpr int FLUSH_PAIRS = 128;  // pending pairs that trigger evaluation
+
 // per-wavefront LDS region
 struct WaveMem {
     float* ts;   // [64*NS] sample depth, +inf past the last sample
-    int* ck;     // [64] stroke index of the c-th hit (compositing order)
-    int* ci0;    // [64] first sample of its interval
-    int* coff;   // [64] offset of its first pair within the chunk
-    int* ccnt;   // [64] its number of pairs
+    int* ck;     // [HMAX] stroke index of the c-th pending hit (compositing order)
+    int* ci0;    // [HMAX] first sample of its interval
+    int* coff;   // [HMAX] offset of its first pair among the pending pairs
+    int* ccnt;   // [HMAX] its number of pairs
     float* pb;   // [npf][PBUF] per-pair values of the current window
     float* scr;  // [64][ng+1] per-lane gradient slots (adjoint)
 };
 
 __host__ __device__ constexpr int wave_mem_floats(int ns, int npf, int ng) {
-    return 64 * ns + 4 * 64 + npf * PBUF + (ng > 0 ? 64 * (ng + 1) : 0);
+    return 64 * ns + 4 * HMAX + npf * PBUF + (ng > 0 ? 64 * (ng + 1) : 0);
 }
 
 __device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf) {
@@ -385,10 +392,10 @@ __device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf) {
     w.ts = base;
     int* ib = (int*)(base + 64 * ns);
     w.ck = ib;
-    w.ci0 = ib + 64;
-    w.coff = ib + 128;
-    w.ccnt = ib + 192;
-    w.pb = base + 64 * ns + 256;
+    w.ci0 = ib + HMAX;
+    w.coff = ib + 2 * HMAX;
+    w.ccnt = ib + 3 * HMAX;
+    w.pb = base + 64 * ns + 4 * HMAX;
     w.scr = w.pb + npf * PBUF;
     return w;
 }
@@ -442,10 +449,11 @@ __device__ __forceinline__ bool stage_depths(const RenderArgs& A, int64_t ray, i
 }
 
 // Culls the 64 strokes of the chunk starting at base0 (in compositing order)
-// and writes the compacted hits to LDS.  Returns the number of hits.
+// and appends the compacted hits to the pending list after its nh hits / np
+// pairs.  Returns the number of new hits; tot receives their number of pairs.
 template <int NS>
-__device__ __forceinline__ int collect_hits(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
-                                            int base0, int lane) {
+__device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
+                                           int base0, int lane, int nh, int np, int& tot) {
     const int k = A.S - 1 - (base0 + lane);
     float ta = 0.f, tb = -1.f;
     bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi, ta, tb);
@@ -462,19 +470,21 @@ __device__ __forceinline__ int collect_hits(const RenderArgs& A, const RayConst&
     const unsigned long long mask = __ballot(hit);
     const int incl = wave_incl_scan(hit ? cnt : 0, lane);
     if (hit) {
-        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        const int rank = nh + __popcll(mask & ((1ull << lane) - 1ull));
         w.ck[rank] = k;
         w.ci0[rank] = i0;
-        w.coff[rank] = incl - cnt;
+        w.coff[rank] = np + incl - cnt;
         w.ccnt[rank] = cnt;
     }
+    tot = __shfl(incl, 63, 64);
     wave_sync();
     return __popcll(mask);
 }
 
-// Largest window [c0, c1) of hits whose pairs fit the staging buffer.
-__device__ __forceinline__ int window_end(const WaveMem& w, int c0, int nc, int wbase, int lane) {
-    const bool fits = lane >= c0 && lane < nc && w.coff[lane] + w.ccnt[lane] - wbase <= PBUF;
+// Largest window [c0, c1) of pending hits (c1 <= nh, c1 - c0 <= 64) whose pairs fit the staging buffer.
+__device__ __forceinline__ int window_end(const WaveMem& w, int c0, int nh, int wbase, int lane) {
+    const int c = c0 + lane;
+    const bool fits = c < nh && w.coff[c] + w.ccnt[c] - wbase <= PBUF;
     return c0 + __popcll(__ballot(fits));
 }
 
@@ -508,35 +518,41 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
     bool full = false;
     const int64_t rh = ray * (int64_t)A.cap_h;
     const int64_t rp = ray * (int64_t)A.cap_p * NPF;
-    for (int base0 = 0; base0 < A.S; base0 += 64) {
-        const int nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
-        bool keep = false;
-        const int chunk_poff = poff;
+    int nh = 0, np = 0;               // pending hits / pairs
+    int kept_np = 0, group_poff = 0;  // pending pairs that are kept (a prefix) and their offset in the region
+    for (int ch = 0; ch < A.nch; ch++) {
+        if (nh == 0) group_poff = poff;
+        int tot;
+        const int nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
         if (A.hdir) {
-            const int tot = nc ? w.coff[nc - 1] + w.ccnt[nc - 1] : 0;
-            keep = !full && hoff + nc <= A.cap_h && poff + tot <= A.cap_p;
+            const bool keep = !full && hoff + nc <= A.cap_h && poff + tot <= A.cap_p;
             full = !keep;
             if (lane == 0) {
-                int* e = A.hdir + (ray * A.nch + base0 / 64) * 3;
+                int* e = A.hdir + (ray * A.nch + ch) * 3;
                 e[0] = keep ? hoff : -1;
                 e[1] = poff;
                 e[2] = nc;
             }
             if (keep && lane < nc) {
                 int* h = A.hbuf + (rh + hoff + lane) * 4;
-                h[0] = w.ck[lane];
-                h[1] = w.ci0[lane];
-                h[2] = w.ccnt[lane];
-                h[3] = w.coff[lane];
+                h[0] = w.ck[nh + lane];
+                h[1] = w.ci0[nh + lane];
+                h[2] = w.ccnt[nh + lane];
+                h[3] = w.coff[nh + lane] - np;  // offset within the chunk
             }
             if (keep) {
                 hoff += nc;
                 poff += tot;
+                kept_np += tot;
             }
         }
-        for (int c0 = 0; c0 < nc;) {
+        nh += nc;
+        np += tot;
+        if (nh <= HMAX - 64 && np < FLUSH_PAIRS && ch + 1 < A.nch) continue;
+        // evaluate and composite the pending hits, window by window
+        for (int c0 = 0; c0 < nh;) {
             const int wbase = w.coff[c0];
-            const int c1 = window_end(w, c0, nc, wbase, lane);
+            const int c1 = window_end(w, c0, nh, wbase, lane);
             const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
             // alpha of every (sample, stroke) pair of the window, one pair per lane
             for (int p0 = 0; p0 < wp; p0 += 64) {
@@ -550,15 +566,16 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
                     sample_point(A, rc, w.ts[i], x, r);
                     const float* rec = A.table + (int64_t)k * REC;
                     const Pair q = eval_pair(rec, ibits(rec[R_BASE]), ibits(rec[R_CURVE]) != 0, x, r, A);
+                    const bool keep = wbase + p < kept_np;
                     w.pb[p] = q.alpha;
-                    if (keep) A.pbuf[rp + chunk_poff + wbase + p] = q.alpha;
+                    if (keep) A.pbuf[rp + group_poff + wbase + p] = q.alpha;
                     if (CID == 1) {
                         const V3 col = color_eval_t<1>(q.p, rc.vd, A.cparams + (int64_t)k * A.Q);
                         w.pb[PBUF + p] = col.x;
                         w.pb[2 * PBUF + p] = col.y;
                         w.pb[3 * PBUF + p] = col.z;
                         if (keep) {
-                            float* kp = A.pbuf + rp + chunk_poff + wbase + p;
+                            float* kp = A.pbuf + rp + group_poff + wbase + p;
                             kp[A.cap_p] = col.x;
                             kp[2 * (int64_t)A.cap_p] = col.y;
                             kp[3 * (int64_t)A.cap_p] = col.z;
@@ -602,6 +619,7 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
             wave_sync();
             c0 = c1;
         }
+        nh = np = kept_np = 0;
     }
     if (counters && lane < NUM_BASE) {
         if (my_pos) atomicAdd(counters + lane, my_pos);
@@ -821,38 +839,59 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
             D2[s] = 0.f;
             C2[s] = v3s(0.f);
         }
-        // ---- strokes in compositing order
+        // ---- strokes in compositing order: pending hits are either all kept by the
+        //      forward (their pair values are contiguous from gpoff) or all recomputed
         const int64_t rh = ray * (int64_t)A.cap_h;
         const int64_t rp = ray * (int64_t)A.cap_p * (CID == 1 ? 4 : 1);
-        for (int base0 = 0; base0 < A.S; base0 += 64) {
-            // hits of this chunk: kept by the forward, or culled again
-            int nc = 0, kpoff = -1;
+        int nh = 0, np = 0, gpoff = 0;
+        bool gkept = false;
+        for (int ch = 0; ch < A.nch;) {
+            int hoff = -1, kpoff = 0, knc = 0;
             if (A.hdir) {
-                const int* e = A.hdir + (ray * A.nch + base0 / 64) * 3;
-                const int hoff = __builtin_amdgcn_readfirstlane(e[0]);
-                if (hoff >= 0) {
-                    kpoff = __builtin_amdgcn_readfirstlane(e[1]);
-                    nc = __builtin_amdgcn_readfirstlane(e[2]);
+                const int* e = A.hdir + (ray * A.nch + ch) * 3;
+                hoff = __builtin_amdgcn_readfirstlane(e[0]);
+                kpoff = __builtin_amdgcn_readfirstlane(e[1]);
+                knc = __builtin_amdgcn_readfirstlane(e[2]);
+            }
+            const bool kept = hoff >= 0;
+            bool do_flush = nh > 0 && kept != gkept;  // evaluate the pending hits before switching source
+            if (!do_flush) {
+                if (nh == 0) {
+                    gkept = kept;
+                    gpoff = kpoff;
+                }
+                int nc, tot;
+                if (kept) {
+                    nc = knc;
+                    int end = 0;
                     if (lane < nc) {
                         const int* h = A.hbuf + (rh + hoff + lane) * 4;
-                        w.ck[lane] = h[0];
-                        w.ci0[lane] = h[1];
-                        w.ccnt[lane] = h[2];
-                        w.coff[lane] = h[3];
+                        w.ck[nh + lane] = h[0];
+                        w.ci0[nh + lane] = h[1];
+                        w.ccnt[nh + lane] = h[2];
+                        w.coff[nh + lane] = np + h[3];
+                        end = h[3] + h[2];
                     }
+                    tot = nc > 0 ? __shfl(end, nc - 1, 64) : 0;
                     wave_sync();
+                } else {
+                    nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
                 }
+                nh += nc;
+                np += tot;
+                ch++;
+                do_flush = nh > HMAX - 64 || np >= FLUSH_PAIRS || ch == A.nch;
             }
-            if (kpoff < 0) nc = collect_hits<NS>(A, rc, w, sorted, base0, lane);
-            for (int c0 = 0; c0 < nc;) {
+            if (!do_flush) continue;
+            for (int c0 = 0; c0 < nh;) {
                 const int wbase = w.coff[c0];
-                const int c1 = window_end(w, c0, nc, wbase, lane);
+                const int c1 = window_end(w, c0, nh, wbase, lane);
                 const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
                 // (1) alpha (and gradient_rgb colour) per pair: kept values or recomputed
-                for (int p0 = 0; kpoff >= 0 && p0 < wp; p0 += 64) {
+                for (int p0 = 0; gkept && p0 < wp; p0 += 64) {
                     const int p = p0 + lane;
                     if (p < wp) {
-                        const float* kp = A.pbuf + rp + kpoff + wbase + p;
+                        const float* kp = A.pbuf + rp + gpoff + wbase + p;
                         w.pb[p] = kp[0];
                         if (CID == 1) {
                             w.pb[PBUF + p] = kp[A.cap_p];
@@ -861,7 +900,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                         }
                     }
                 }
-                for (int p0 = 0; kpoff < 0 && p0 < wp; p0 += 64) {
+                for (int p0 = 0; !gkept && p0 < wp; p0 += 64) {
                     const int p = p0 + lane;
                     if (p < wp) {
                         const int c = find_hit(w, c0, c1, wbase, p);
@@ -1016,6 +1055,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                 }
                 c0 = c1;
             }
+            nh = np = 0;
         }
     }
     if (use_lds) {

@@ -94,6 +94,194 @@ def accuracy_block(args, sc, T, ids, gr, ga, n):
             'fused_vs_fp64': errs(fused), 'plain_fp32_vs_fp64': errs(p32), 'fused_vs_plain_fp32': f_vs_p}
 
 
+def fd_block(sc, T, ids, gr, ga, n_rays=512, n_params=32, h=1e-4, seed=11):
+    """Central finite differences (float64 plain formulation on the GPU, step h) on
+    n_params random shape / colour / density parameters of strokes the rays touch,
+    against the fused float32 exact adjoint on the same rays."""
+    import torch
+    from neural3dstrokes_amd import plain, render
+    sl = slice(0, n_rays)
+    d64 = torch.float64
+    args = [T[k][sl].to(d64) for k in ('origins', 'directions', 'radii', 'tdist')]
+
+    def loss(sp, cp, dp):
+        out = torch.zeros((), dtype=d64, device=sp.device)
+        for r0 in range(0, n_rays, 128):
+            s2 = slice(r0, r0 + 128)
+            rgb, acc = plain.field_chunk(args[0][s2], args[1][s2], args[2][s2], args[3][s2], groups, sp, cp, dp)
+            out = out + (rgb * gr[sl][s2].to(d64)).sum() + (acc * ga[sl][s2].to(d64)).sum()
+        return out.item()
+
+    ids_l = sc['stroke_ids'].tolist()
+    dev = T['shape_params'].device
+    groups = [(sid, torch.tensor([k for k in range(len(ids_l)) if ids_l[k] == sid], device=dev))
+              for sid in sorted(set(ids_l))]
+    sp = T['shape_params'].clone().requires_grad_(True)
+    cp = T['color_params'].clone().requires_grad_(True)
+    dp = T['density_params'].clone().requires_grad_(True)
+    rgb, acc, _ = render.render_strokes(T['origins'][sl], T['directions'][sl], T['viewdirs'][sl], T['radii'][sl],
+                                        T['tdist'][sl], ids, sp, cp, dp, adjoint='exact')
+    torch.autograd.backward([rgb, acc], [gr[sl], ga[sl]])
+    touched = ((sp.grad.abs().sum(1) + cp.grad.abs().sum(1) + dp.grad.abs()) > 0).nonzero().flatten().tolist()
+    g = torch.Generator().manual_seed(seed)
+    from neural3dstrokes_amd.scenes import n_shape_params
+    cands = []
+    for k in touched:
+        cands += [('shape', k, j) for j in range(n_shape_params(ids_l[k]))] + [('color', k, j) for j in range(3)] + \
+                 [('density', k, 0)]
+    pick = torch.randperm(len(cands), generator=g)[:n_params].tolist()
+    base = [T['shape_params'].to(d64), T['color_params'].to(d64), T['density_params'].to(d64)]
+    fd, an = [], []
+    for i in pick:
+        kind, k, j = cands[i]
+        which = {'shape': 0, 'color': 1, 'density': 2}[kind]
+        vals = []
+        for sgn in (1.0, -1.0):
+            p = [b.clone() for b in base]
+            if which == 2:
+                p[2][k] += sgn * h
+            else:
+                p[which][k, j] += sgn * h
+            vals.append(loss(*p))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+        an.append(float(sp.grad[k, j] if which == 0 else (cp.grad[k, j] if which == 1 else dp.grad[k])))
+    fd_t, an_t = torch.tensor(fd, dtype=d64), torch.tensor(an, dtype=d64)
+    per = ((an_t - fd_t).abs() / fd_t.abs().clamp_min(1e-12)).tolist()
+    return {'rays': n_rays, 'params': len(pick), 'h': h, 'function': 'plain formulation in float64 on the GPU',
+            'rel_err_norm': ((an_t - fd_t).norm() / fd_t.norm().clamp_min(1e-30)).item(),
+            'rel_err_median': float(torch.tensor(per).median()), 'within_1e-3': sum(e < 1e-3 for e in per),
+            'within_1e-2': sum(e < 1e-2 for e in per)}
+
+
+def grid_block(seed=0, n_points=1 << 20, steps=20):
+    """Hash-grid encoder (16 levels x 2 features, 2^19 rows per level, resolution 16 -> 2048)
+    fwd / bwd kernel times on 2^20 points, fp32 and fp16 tables, GB/s against 8 TB/s and
+    errors against the float64 oracle on 4096 points."""
+    import numpy as np
+    import torch
+    from neural3dstrokes_amd import _lib
+    from neural3dstrokes_amd.gridencoder import GridEncoder
+    from oracle import grid as ogrid
+    dev = torch.device('cuda:0')
+    enc = GridEncoder(3, 16, 2, base_resolution=16, log2_hashmap_size=19, desired_resolution=2048).to(dev)
+    g = torch.Generator().manual_seed(seed)
+    x = torch.rand(n_points, 3, generator=g).to(dev)
+    emb32 = (torch.rand(enc.embeddings.shape, generator=g) * 2e-4 - 1e-4).to(dev)
+    gout = torch.randn(n_points, 32, generator=g).to(dev)
+    L, C, D, H = 16, 2, 3, 16
+    S = float(np.log2(enc.per_level_scale))
+    rows = emb32.shape[0]
+    res = {}
+    lib = _lib.lib()
+    for name, dt, flag in (('fp32', torch.float32, 0), ('fp16', torch.float16, 1)):
+        emb = emb32.to(dt).contiguous()
+        out = torch.empty(n_points, L * C, dtype=dt, device=dev)
+        gin = gout.to(dt).contiguous()
+        g32 = torch.zeros(rows, C, dtype=torch.float32, device=dev)
+        gcast = torch.empty(rows, C, dtype=dt, device=dev) if flag else g32
+        nws = int(lib.n3ds_grid_encode_backward_workspace_bytes(n_points, D, C, L, rows))
+        ws = torch.empty(nws, dtype=torch.uint8, device=dev)
+
+        def fwd():
+            _lib.check(lib.n3ds_grid_encode_forward(_lib.ptr(x), _lib.ptr(emb), _lib.ptr(enc.offsets), _lib.ptr(out),
+                                                    n_points, D, C, L, S, H, None, 0, 0, 0, flag, _lib.stream()))
+
+        def bwd(wsp=ws):
+            _lib.check(lib.n3ds_grid_encode_backward(_lib.ptr(gin), _lib.ptr(x), _lib.ptr(enc.offsets), _lib.ptr(g32),
+                                                     _lib.ptr(gcast), rows, n_points, D, C, L, S, H, None, None, 0, 0,
+                                                     0, flag, _lib.ptr(wsp), nws if wsp is not None else 0,
+                                                     _lib.stream()))
+
+        times = {}
+        for kname, fn in (('fwd', fwd), ('bwd', bwd), ('bwd_atomic', lambda: bwd(None))):
+            for _ in range(3):
+                g32.zero_()
+                fn()
+            evs = []
+            for _ in range(steps):
+                g32.zero_()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                fn()
+                e1.record()
+                evs.append((e0, e1))
+            torch.cuda.synchronize()
+            times[kname] = sum(a.elapsed_time(b) for a, b in evs) / steps
+        es = 2 if flag else 4
+        b_fwd = n_points * (12 + L * C * es) + rows * C * es
+        b_bwd = n_points * (12 + L * C * es) + 2 * rows * C * 4 + (rows * C * (4 + es) if flag else 0)
+        # errors against the float64 oracle on the first 4096 points
+        xs = x[:4096].double().cpu().numpy()
+        ref_out, _ = ogrid.grid_forward(xs, emb.double().cpu().numpy(), enc.offsets.cpu().numpy(), enc.per_level_scale,
+                                        16)
+        ref_g, _ = ogrid.grid_backward(gin[:4096].double().cpu().numpy(), xs, emb.double().cpu().numpy(),
+                                       enc.offsets.cpu().numpy(), enc.per_level_scale, 16)
+        out_s = out[:4096].double().cpu().numpy()
+        g32.zero_()
+        _lib.check(lib.n3ds_grid_encode_backward(_lib.ptr(gin[:4096].contiguous()), _lib.ptr(x[:4096].contiguous()),
+                                                 _lib.ptr(enc.offsets), _lib.ptr(g32), _lib.ptr(gcast), rows, 4096, D,
+                                                 C, L, S, H, None, None, 0, 0, 0, flag, _lib.ptr(ws), nws,
+                                                 _lib.stream()))
+        torch.cuda.synchronize()
+        gg = g32.double().cpu().numpy()
+        res[name] = {
+            'fwd_ms': times['fwd'], 'bwd_ms': times['bwd'], 'bwd_ms_float_atomics': times['bwd_atomic'],
+            'bwd_workspace_bytes': nws,
+            'fwd_bytes': b_fwd, 'bwd_bytes': b_bwd,
+            'fwd_GBps': b_fwd / (times['fwd'] * 1e-3) / 1e9, 'bwd_GBps': b_bwd / (times['bwd'] * 1e-3) / 1e9,
+            'fwd_frac_hbm': b_fwd / (times['fwd'] * 1e-3) / 8e12, 'bwd_frac_hbm': b_bwd / (times['bwd'] * 1e-3) / 8e12,
+            'rel_err_vs_fp64': {'outputs': float(np.linalg.norm(out_s - ref_out) / np.linalg.norm(ref_out)),
+                                'grad_embeddings_fp32_accum': float(np.linalg.norm(gg - ref_g) / np.linalg.norm(ref_g))},
+        }
+    return {'workload': f'{n_points} points, 16 levels x 2 features, 2^19 rows/level, res 16->2048, trilinear',
+            'bytes_model': 'compulsory: inputs 12 B/pt + outputs (or upstream grads) L*C*elem B/pt + table once '
+                           '(bwd: fp32 grad table zero-init + write, fp16 adds the cast pass)',
+            'peak_GBps': 8000.0, **res}
+
+
+def memory_block(steps=3, warmup=1):
+    """640,000 rays x 192 samples x 1000 strokes: fused fwd+bwd with the forward's hits / alpha
+    kept for the adjoint versus full recomputation; throughput and peak HBM for both."""
+    import torch
+    from neural3dstrokes_amd import render, scenes
+    dev = torch.device('cuda:0')
+    sc = scenes.make_workload('cfg4', seed=0)
+    R, n1 = sc['tdist'].shape
+    g = torch.Generator().manual_seed(1)
+    gr, ga = torch.randn(R, 3, generator=g).to(dev), torch.randn(R, generator=g).to(dev)
+    T = {k: sc[k].to(dev) for k in ('origins', 'directions', 'viewdirs', 'radii', 'tdist', 'shape_params',
+                                     'color_params', 'density_params')}
+    ids = sc['stroke_ids'].to(dev)
+    out = {'workload': f'{len(ids)} strokes x {R} rays x {n1 - 1} samples, exact adjoint'}
+    for mode in (False, True):
+        sp = T['shape_params'].clone().requires_grad_(True)
+        cp = T['color_params'].clone().requires_grad_(True)
+        dp = T['density_params'].clone().requires_grad_(True)
+
+        def step():
+            sp.grad = cp.grad = dp.grad = None
+            rgb, acc, _ = render.render_strokes(T['origins'], T['directions'], T['viewdirs'], T['radii'], T['tdist'],
+                                                ids, sp, cp, dp, store=mode)
+            torch.autograd.backward([rgb, acc], [gr, ga])
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out['kept' if mode else 'recompute'] = {'ms_per_step': ms, 'ray_samples_per_s': R * (n1 - 1) / (ms * 1e-3),
+                                                'peak_step_bytes': torch.cuda.max_memory_allocated() - base}
+        del sp, cp, dp
+    return out
+
+
 def cpu_baseline(sc, n_rays, g_rgb, g_acc):
     """Float32 CPU oracle (plain per-op math, no culling) fwd+bwd on the first n_rays rays."""
     import torch
@@ -129,6 +317,9 @@ def main():
     ap.add_argument('--cpu-rays', type=int, default=256, help='rays for the CPU baseline (0: skip)')
     ap.add_argument('--fused-only', action='store_true', help='only the fused steps (for profiler runs)')
     ap.add_argument('--acc-rays', type=int, default=4096, help='rays for the float64 accuracy check (0: skip)')
+    ap.add_argument('--fd-params', type=int, default=32, help='parameters for the finite-difference check (0: skip)')
+    ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
+    ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build measures one GPU; multi-GPU runs are not part of it',
@@ -251,6 +442,12 @@ def main():
                                                    'grad_density': rel(dpf.grad, pgd)}}
         if args.acc_rays > 0:
             out['accuracy'] = accuracy_block(args, sc, T, ids, gr, ga, n)
+    if args.fd_params > 0:
+        out['finite_differences'] = fd_block(sc, T, ids, gr, ga, n_params=args.fd_params)
+    if args.grid:
+        out['hash_grid'] = grid_block()
+    if args.memory:
+        out['memory_bounded'] = memory_block()
     if args.cpu_rays > 0:
         out['cpu_baseline'] = cpu_baseline(sc, min(args.cpu_rays, R), g_rgb, g_acc)
     print(json.dumps(out))

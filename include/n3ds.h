@@ -111,11 +111,21 @@ int n3ds_grid_encode_forward(const float* inputs, const void* embeddings, const 
  * grad [B, L*C] of `dtype`; grad_embeddings_f32 [n_rows,C] float32 is
  * accumulated (zero-initialised by the caller); grad_embeddings_out
  * [n_rows,C] of `dtype` receives the rounded result when dtype = 1 (may be
- * NULL); grad_inputs [B,D] (written) needs dy_dx, both may be NULL. */
+ * NULL); grad_inputs [B,D] (written) needs dy_dx, both may be NULL.
+ * workspace (device, at least n3ds_grid_encode_backward_workspace_bytes):
+ * the table gradient is scattered by bucket of consecutive rows and summed in
+ * LDS, with no float atomics to global memory on the hot path; NULL: one
+ * float atomic per (point, level, corner, feature), as the reference does. */
 int n3ds_grid_encode_backward(const void* grad, const float* inputs, const int32_t* offsets,
                               float* grad_embeddings_f32, void* grad_embeddings_out, int64_t n_rows, int64_t B,
                               int D, int C, int L, float S, int H, const void* dy_dx, void* grad_inputs,
-                              int gridtype, int align_corners, int interp, int dtype, void* stream);
+                              int gridtype, int align_corners, int interp, int dtype, void* workspace,
+                              int64_t workspace_bytes, void* stream);
+
+/* Bytes of device workspace the bucketed backward needs (host-side size
+ * query; -1 for invalid sizes).  No counterpart in the reference (its
+ * backward scatters with atomics only). */
+int64_t n3ds_grid_encode_backward_workspace_bytes(int64_t B, int D, int C, int L, int64_t n_rows);
 
 /* Replaces the pybind `grad_total_variation` of gridencoder.h:15
  * (kernel_grad_tv gridencoder.cu:506-645), called from grid.py:172-194.

@@ -31,7 +31,8 @@ SIGNATURES = {
     'n3ds_grid_encode_forward': (_I32, [_P, _P, _P, _P, _I64, _I32, _I32, _I32, _F, _I32, _P, _I32, _I32, _I32, _I32,
                                         _P]),
     'n3ds_grid_encode_backward': (_I32, [_P, _P, _P, _P, _P, _I64, _I64, _I32, _I32, _I32, _F, _I32, _P, _P, _I32, _I32,
-                                         _I32, _I32, _P]),
+                                         _I32, _I32, _P, _I64, _P]),
+    'n3ds_grid_encode_backward_workspace_bytes': (_I64, [_I64, _I32, _I32, _I32, _I64]),
     'n3ds_grid_grad_tv': (_I32, [_P, _P, _P, _P, _F, _I64, _I32, _I32, _I32, _F, _I32, _I32, _I32, _I32, _P]),
     'n3ds_render_table_floats': (_I64, [_I32]),
     'n3ds_render_prepare': (_I32, [_P, _P, _P, _I32, _I32, _P, _P]),

@@ -229,6 +229,11 @@ __global__ void grid_input_bwd_kernel(const T* __restrict__ grad, const T* __res
     gin[t] = from_f<T>(r);
 }
 
+__global__ void grid_fill_int_kernel(int* __restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 template <typename T>
 __global__ void f32_to_kernel(const float* __restrict__ src, T* __restrict__ dst, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -285,6 +290,223 @@ __global__ void grid_tv_kernel(const float* __restrict__ inputs, const T* __rest
     for (int c = 0; c < C; c++) atomicAdd(gr + idx + c, w * res[c] * rsqrtf(idel[c] + 1e-9f));
 }
 
+// ---------------------------------------------------------------------------
+// Bucketed gradient scatter (no float atomics to global memory on the hot path).
+// Device-scope float atomics run at a fraction of HBM speed, so the adjoint
+// groups its (row, value) contributions by bucket of RB consecutive table rows
+// and sums each bucket in LDS:
+//   1. count:   per (level, point tile) block, an LDS histogram of the buckets
+//               its contributions fall into, added to the global bucket counts;
+//   2. scan:    bucket starts and, per bucket, the number of SEG-record segments;
+//   3. scatter: the same contributions again, written as (row, C values) records
+//               into per-bucket ranges reserved with one atomic per (block, bucket);
+//   4. reduce:  one block per bucket segment accumulates its records into an
+//               LDS copy of the bucket's rows and adds it to the gradient table
+//               (plain read-modify-write when the bucket has one segment).
+// ---------------------------------------------------------------------------
+constexpr int GB_THREADS = 256;
+constexpr int GB_PPT = 8;           // points per thread in the count / scatter passes
+constexpr int GB_HBINS = 4096;      // LDS histogram bins per block (buckets one level spans)
+constexpr int64_t GB_SEG = 262144;  // records per reduce block
+
+__host__ __device__ constexpr int gb_rows(int C) { return 16384 / C; }  // RB: 64 KB of fp32 rows in LDS
+
+struct GridBucketWs {
+    int* count;      // [NB]
+    int* start;      // [NB + 1]
+    int* cursor;     // [NB]
+    int* segstart;   // [NB + 1]
+    uint32_t* rows;  // [records]
+    float* vals;     // [records * C]
+};
+
+static int64_t gb_num_buckets(int64_t n_rows, int C) { return (n_rows + gb_rows(C) - 1) / gb_rows(C); }
+
+static int64_t gb_workspace_bytes(int64_t B, int D, int C, int L, int64_t n_rows) {
+    const int64_t NB = gb_num_buckets(n_rows, C);
+    const int64_t records = B * L * (1ll << D);
+    int64_t head = (4 * NB + 2) * 4;
+    head = (head + 255) / 256 * 256;
+    return head + records * 4 + records * C * 4;
+}
+
+static GridBucketWs gb_carve(void* ws, int64_t B, int D, int C, int L, int64_t n_rows) {
+    const int64_t NB = gb_num_buckets(n_rows, C);
+    const int64_t records = B * L * (1ll << D);
+    GridBucketWs w;
+    int* ip = (int*)ws;
+    w.count = ip;
+    w.start = ip + NB;
+    w.cursor = ip + 2 * NB + 1;
+    w.segstart = ip + 3 * NB + 1;
+    int64_t head = (4 * NB + 2) * 4;
+    head = (head + 255) / 256 * 256;
+    w.rows = (uint32_t*)((char*)ws + head);
+    w.vals = (float*)((char*)ws + head + records * 4);
+    return w;
+}
+
+// PASS 0: count, PASS 1: scatter.  blockIdx.y = level, blockIdx.x = tile of GB_THREADS * GB_PPT points.
+template <typename T, int D, int C, int PASS>
+__global__ __launch_bounds__(GB_THREADS) void grid_bucket_kernel(const T* __restrict__ grad,
+                                                                 const float* __restrict__ inputs,
+                                                                 const int* __restrict__ offsets, int64_t B, int L,
+                                                                 LevelTab lt, int gridtype, int align, int interp,
+                                                                 GridBucketWs ws) {
+    constexpr int RB = gb_rows(C);
+    __shared__ int hist[GB_HBINS];
+    __shared__ int base[GB_HBINS];
+    const int level = blockIdx.y;
+    const int row0 = offsets[level], row1 = offsets[level + 1];
+    const int b_lo = row0 / RB;
+    const int nbins = row1 > row0 ? (row1 - 1) / RB - b_lo + 1 : 0;
+    const bool local = nbins <= GB_HBINS;
+    for (int i = threadIdx.x; i < GB_HBINS; i += GB_THREADS) hist[i] = 0;
+    __syncthreads();
+    const int64_t p_begin = (int64_t)blockIdx.x * GB_THREADS * GB_PPT;
+    // pass 1 needs the histogram before it can place records: count first (both passes)
+#pragma unroll 1
+    for (int it = 0; it < GB_PPT; it++) {
+        const int64_t b = p_begin + (int64_t)it * GB_THREADS + threadIdx.x;
+        if (b >= B) break;
+        float pos[D], deriv[D];
+        uint32_t pg[D];
+        LevelCoord lc = level_coords<D>(inputs + b * D, level, offsets, lt, align != 0, interp, pos, deriv, pg);
+        if (lc.oob) continue;
+#pragma unroll
+        for (int corner = 0; corner < (1 << D); corner++) {
+            uint32_t loc[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) loc[d] = (corner & (1 << d)) ? pg[d] + 1 : pg[d];
+            const int row = row0 + (int)grid_index<D>(gridtype, align != 0, lc.hs, lc.res, loc);
+            const int bk = row / RB;
+            if (local) atomicAdd(&hist[bk - b_lo], 1);
+            else if (PASS == 0) atomicAdd(&ws.count[bk], 1);
+        }
+    }
+    __syncthreads();
+    if (PASS == 0) {
+        if (local)
+            for (int i = threadIdx.x; i < nbins; i += GB_THREADS)
+                if (hist[i]) atomicAdd(&ws.count[b_lo + i], hist[i]);
+        return;
+    }
+    // pass 1: reserve this block's range in each bucket, then write the records
+    if (local) {
+        for (int i = threadIdx.x; i < nbins; i += GB_THREADS) {
+            base[i] = hist[i] ? atomicAdd(&ws.cursor[b_lo + i], hist[i]) : 0;
+            hist[i] = 0;
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int it = 0; it < GB_PPT; it++) {
+        const int64_t b = p_begin + (int64_t)it * GB_THREADS + threadIdx.x;
+        if (b >= B) break;
+        float pos[D], deriv[D];
+        uint32_t pg[D];
+        LevelCoord lc = level_coords<D>(inputs + b * D, level, offsets, lt, align != 0, interp, pos, deriv, pg);
+        if (lc.oob) continue;
+        float gcur[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) gcur[c] = to_f<T>(grad[(b * L + level) * C + c]);
+#pragma unroll
+        for (int corner = 0; corner < (1 << D); corner++) {
+            float wgt = 1.f;
+            uint32_t loc[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                if (corner & (1 << d)) { wgt *= pos[d]; loc[d] = pg[d] + 1; }
+                else { wgt *= 1.f - pos[d]; loc[d] = pg[d]; }
+            }
+            const int row = row0 + (int)grid_index<D>(gridtype, align != 0, lc.hs, lc.res, loc);
+            const int bk = row / RB;
+            const int64_t slot = local ? (int64_t)base[bk - b_lo] + atomicAdd(&hist[bk - b_lo], 1)
+                                       : (int64_t)atomicAdd(&ws.cursor[bk], 1);
+            ws.rows[slot] = (uint32_t)row;
+#pragma unroll
+            for (int c = 0; c < C; c++) ws.vals[slot * C + c] = wgt * gcur[c];
+        }
+    }
+}
+
+// bucket starts, reservation cursors and per-bucket segment offsets (one block)
+__global__ __launch_bounds__(1024) void grid_bucket_scan_kernel(GridBucketWs ws, int64_t NB) {
+    __shared__ int sa[1024], sb[1024];
+    __shared__ int carry_a, carry_b;
+    if (threadIdx.x == 0) { carry_a = 0; carry_b = 0; }
+    __syncthreads();
+    for (int64_t t0 = 0; t0 < NB; t0 += 1024) {
+        const int64_t i = t0 + threadIdx.x;
+        const int cnt = i < NB ? ws.count[i] : 0;
+        const int nseg = (int)((cnt + GB_SEG - 1) / GB_SEG);
+        sa[threadIdx.x] = cnt;
+        sb[threadIdx.x] = nseg;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int va = threadIdx.x >= off ? sa[threadIdx.x - off] : 0;
+            int vb = threadIdx.x >= off ? sb[threadIdx.x - off] : 0;
+            __syncthreads();
+            sa[threadIdx.x] += va;
+            sb[threadIdx.x] += vb;
+            __syncthreads();
+        }
+        if (i < NB) {
+            const int ea = carry_a + sa[threadIdx.x] - cnt;
+            ws.start[i] = ea;
+            ws.cursor[i] = ea;
+            ws.segstart[i] = carry_b + sb[threadIdx.x] - nseg;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) {
+            carry_a += sa[1023];
+            carry_b += sb[1023];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ws.start[NB] = carry_a;
+        ws.segstart[NB] = carry_b;
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(GB_THREADS) void grid_bucket_reduce_kernel(GridBucketWs ws, int64_t NB, int64_t n_rows,
+                                                                        float* __restrict__ ggrid) {
+    constexpr int RB = gb_rows(C);
+    __shared__ float acc[RB * C];
+    const int64_t j = blockIdx.x;
+    if (j >= ws.segstart[NB]) return;
+    // bucket of segment j: last bucket with segstart <= j
+    int64_t lo = 0, hi = NB - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (ws.segstart[mid] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t bk = lo;
+    const int nseg = ws.segstart[bk + 1] - ws.segstart[bk];
+    const int64_t r0 = ws.start[bk] + (j - ws.segstart[bk]) * GB_SEG;
+    const int64_t r1 = min(r0 + GB_SEG, (int64_t)ws.start[bk + 1]);
+    for (int i = threadIdx.x; i < RB * C; i += GB_THREADS) acc[i] = 0.f;
+    __syncthreads();
+    const int row_base = (int)(bk * RB);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += GB_THREADS) {
+        const int lr = (int)ws.rows[r] - row_base;
+#pragma unroll
+        for (int c = 0; c < C; c++) atomicAdd(&acc[lr * C + c], ws.vals[r * C + c]);
+    }
+    __syncthreads();
+    const int64_t nr = min((int64_t)RB, n_rows - row_base);
+    float* g = ggrid + (int64_t)row_base * C;
+    for (int64_t i = threadIdx.x; i < nr * C; i += GB_THREADS) {
+        const float v = acc[i];
+        if (v == 0.f) continue;
+        if (nseg == 1) g[i] += v;
+        else atomicAdd(&g[i], v);
+    }
+}
+
 #define N3DS_GRID_DISPATCH(KERNEL_LAUNCH)                                                           \
     switch (D * 16 + C) {                                                                          \
         case 2 * 16 + 1: KERNEL_LAUNCH(2, 1); break;                                               \
@@ -335,25 +557,62 @@ N3DS_EXPORT int n3ds_grid_encode_forward(const float* inputs, const void* embedd
     return N3DS_OK;
 }
 
+N3DS_EXPORT int64_t n3ds_grid_encode_backward_workspace_bytes(int64_t B, int D, int C, int L, int64_t n_rows) {
+    if (B < 0 || D < 1 || D > 5 || C < 1 || C > 8 || L < 1 || n_rows < 0) return -1;
+    return gb_workspace_bytes(B, D, C, L, n_rows);
+}
+
 // grad_embeddings_f32: fp32 accumulation table (zero-initialised by the caller);
 // grad_embeddings_out: optional table of the embedding dtype that receives the
 // rounded result (may alias grad_embeddings_f32 when dtype == float32).
+// workspace (workspace_bytes >= n3ds_grid_encode_backward_workspace_bytes):
+// bucketed scatter; NULL: one float atomic per contribution.
 N3DS_EXPORT int n3ds_grid_encode_backward(const void* grad, const float* inputs, const int32_t* offsets,
                                           float* grad_embeddings_f32, void* grad_embeddings_out, int64_t n_rows,
                                           int64_t B, int D, int C, int L, float S, int H, const void* dy_dx,
                                           void* grad_inputs, int gridtype, int align_corners, int interp, int dtype,
-                                          void* stream) {
+                                          void* workspace, int64_t workspace_bytes, void* stream) {
     int st = grid_check(B, D, C, L, dtype);
     const LevelTab lt = make_level_tab(L, S, H);
     if (st) return st;
     hipStream_t s = (hipStream_t)stream;
+    const bool bucketed = workspace != nullptr;
+    if (bucketed)
+        N3DS_REQUIRE(workspace_bytes >= gb_workspace_bytes(B, D, C, L, n_rows), "grid workspace too small");
+    if (B > 0 && bucketed && n_rows > 0) {
+        const GridBucketWs ws = gb_carve(workspace, B, D, C, L, n_rows);
+        const int64_t NB = gb_num_buckets(n_rows, C);
+        hipLaunchKernelGGL(grid_fill_int_kernel, dim3((unsigned)((NB + 255) / 256)), dim3(256), 0, s, ws.count, NB);
+        N3DS_LAUNCH_CHECK();
+        dim3 gpass((unsigned)((B + GB_THREADS * GB_PPT - 1) / (GB_THREADS * GB_PPT)), (unsigned)L);
+        const int64_t max_segs = (B * L * (1ll << D) + GB_SEG - 1) / GB_SEG + NB;
+#define LAUNCH_G(DD, CC)                                                                                          \
+    if (dtype == 0) {                                                                                             \
+        hipLaunchKernelGGL((grid_bucket_kernel<float, DD, CC, 0>), gpass, dim3(GB_THREADS), 0, s,                 \
+                           (const float*)grad, inputs, offsets, B, L, lt, gridtype, align_corners, interp, ws);   \
+        hipLaunchKernelGGL(grid_bucket_scan_kernel, dim3(1), dim3(1024), 0, s, ws, NB);                           \
+        hipLaunchKernelGGL((grid_bucket_kernel<float, DD, CC, 1>), gpass, dim3(GB_THREADS), 0, s,                 \
+                           (const float*)grad, inputs, offsets, B, L, lt, gridtype, align_corners, interp, ws);   \
+    } else {                                                                                                      \
+        hipLaunchKernelGGL((grid_bucket_kernel<__half, DD, CC, 0>), gpass, dim3(GB_THREADS), 0, s,                \
+                           (const __half*)grad, inputs, offsets, B, L, lt, gridtype, align_corners, interp, ws);  \
+        hipLaunchKernelGGL(grid_bucket_scan_kernel, dim3(1), dim3(1024), 0, s, ws, NB);                           \
+        hipLaunchKernelGGL((grid_bucket_kernel<__half, DD, CC, 1>), gpass, dim3(GB_THREADS), 0, s,                \
+                           (const __half*)grad, inputs, offsets, B, L, lt, gridtype, align_corners, interp, ws);  \
+    }                                                                                                             \
+    hipLaunchKernelGGL((grid_bucket_reduce_kernel<CC>), dim3((unsigned)max_segs), dim3(GB_THREADS), 0, s, ws, NB, \
+                       n_rows, grad_embeddings_f32);
+        N3DS_GRID_DISPATCH(LAUNCH_G)
+#undef LAUNCH_G
+        N3DS_LAUNCH_CHECK();
+    }
     if (B > 0) {
         dim3 grid((unsigned)((B * L + 255) / 256));
 #define LAUNCH_B(DD, CC)                                                                                          \
-    if (dtype == 0)                                                                                               \
+    if (!bucketed && dtype == 0)                                                                                  \
         hipLaunchKernelGGL((grid_bwd_kernel<float, DD, CC>), grid, dim3(256), 0, s, (const float*)grad, inputs,   \
                            offsets, grad_embeddings_f32, B, L, lt, gridtype, align_corners, interp);   \
-    else                                                                                                          \
+    else if (!bucketed)                                                                                           \
         hipLaunchKernelGGL((grid_bwd_kernel<__half, DD, CC>), grid, dim3(256), 0, s, (const __half*)grad, inputs, \
                            offsets, grad_embeddings_f32, B, L, lt, gridtype, align_corners, interp);   \
     if (dy_dx && grad_inputs) {                                                                                   \

@@ -15,6 +15,10 @@ from torch.autograd import Function
 from .. import _lib
 
 _gridtype_to_id = {'hash': 0, 'tiled': 1}
+# table-gradient scatter of the backward: 'bucketed' (records grouped by row
+# bucket and summed in LDS; needs a workspace of about (4 + 4C) bytes per
+# (point, level, corner)) or 'atomic' (one float atomic per contribution)
+SCATTER = 'bucketed'
 _interp_to_id = {'linear': 0, 'smoothstep': 1}
 
 
@@ -64,10 +68,13 @@ class _grid_encode(Function):
         g32 = torch.zeros(embeddings.shape, dtype=torch.float32, device=embeddings.device)
         g_out = g32 if embeddings.dtype == torch.float32 else torch.empty_like(embeddings)
         grad_inputs = torch.zeros_like(inputs, dtype=embeddings.dtype) if dy_dx is not None else None
+        rows = embeddings.shape[0]
+        nws = int(_lib.lib().n3ds_grid_encode_backward_workspace_bytes(B, D, C, L, rows))
+        ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=embeddings.device) if SCATTER == 'bucketed' else None
         _lib.check(_lib.lib().n3ds_grid_encode_backward(
-            _lib.ptr(grad), _lib.ptr(inputs), _lib.ptr(offsets), _lib.ptr(g32), _lib.ptr(g_out), embeddings.shape[0],
+            _lib.ptr(grad), _lib.ptr(inputs), _lib.ptr(offsets), _lib.ptr(g32), _lib.ptr(g_out), rows,
             B, D, C, L, S, H, _lib.ptr(dy_dx), _lib.ptr(grad_inputs), int(gridtype), int(bool(align_corners)),
-            int(interpolation), _dtype_flag(embeddings), _lib.stream()))
+            int(interpolation), _dtype_flag(embeddings), _lib.ptr(ws), nws if ws is not None else 0, _lib.stream()))
         if grad_inputs is not None:
             grad_inputs = grad_inputs.to(inputs.dtype)
         return grad_inputs, g_out, None, None, None, None, None, None, None

@@ -19,7 +19,9 @@ def _enc(dev, **kw):
                                 dict(num_levels=8, level_dim=4, log2_hashmap_size=15, gridtype='tiled'),
                                 dict(num_levels=6, level_dim=2, interpolation='smoothstep', align_corners=True),
                                 dict(input_dim=2, num_levels=6, level_dim=1, log2_hashmap_size=12)])
-def test_grid_forward_backward(kw, dev):
+@pytest.mark.parametrize('scatter', ['bucketed', 'atomic'])
+def test_grid_forward_backward(kw, scatter, dev):
+    from neural3dstrokes_amd.gridencoder import grid as dgrid
     enc = _enc(dev, **kw)
     g = torch.Generator().manual_seed(1)
     B = 4096
@@ -27,7 +29,11 @@ def test_grid_forward_backward(kw, dev):
     x = torch.rand(B, D, generator=g) * 2.2 - 1.1  # includes out-of-range points (zero output)
     out = enc(x.to(dev))
     gout = torch.randn(out.shape, generator=g)
-    out.backward(gout.to(dev))
+    saved, dgrid.SCATTER = dgrid.SCATTER, scatter
+    try:
+        out.backward(gout.to(dev))
+    finally:
+        dgrid.SCATTER = saved
     inp = ((x + 1) / 2).numpy()
     ref, _ = ogrid.grid_forward(inp, enc.embeddings.detach().cpu().numpy(), enc.offsets.cpu().numpy(),
                                 enc.per_level_scale, enc.base_resolution, enc.gridtype_id, enc.align_corners,

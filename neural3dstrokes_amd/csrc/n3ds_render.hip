@@ -55,6 +55,7 @@ enum : int {
     R_MINS = 28,    // min scale
     R_E0 = 29,      // 3: unit box half extent
     R_E1 = 32,      // 3: margin multiplier
+    R_M = 36,       // 9: culling map diag(1/s) R^T (row-major)
     R_CP = 8,       // 14: curve params (curves reuse the solid slots)
     R_BOX = 22,     // NPIECE * 6: piece boxes
     R_HULL = 118,   // 6: whole-curve box
@@ -159,6 +160,9 @@ __global__ void render_prep_kernel(float* __restrict__ table, const int* __restr
         rec[R_ANG + 0] = F.angle.x; rec[R_ANG + 1] = F.angle.y; rec[R_ANG + 2] = F.angle.z;
         for (int i = 0; i < nb; i++) rec[R_BP + i] = row[i];
         rec[R_MINS] = fminf(F.scale.x, fminf(F.scale.y, F.scale.z));
+        const float isc[3] = {1.f / F.scale.x, 1.f / F.scale.y, 1.f / F.scale.z};
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < 3; j++) rec[R_M + 3 * a + j] = Rm.r[j][a] * isc[a];
         // unit-space box: points with sdf < m satisfy |p_a| < e0_a + e1_a m
         V3 e0 = v3s(1.f), e1 = v3s(1.f);
         float b0 = nb > 0 ? row[0] : 0.f, b1 = nb > 1 ? row[1] : 0.f;
@@ -248,20 +252,21 @@ __device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o,
         t0 = u0;
         t1 = u1;
     } else {
-        const float* r = rec + R_ROT;
-        V3 tr = ld3(rec + R_TR), sc = ld3(rec + R_SC);
-        V3 q = o - tr;
-        V3 A = v3((r[0] * q.x + r[3] * q.y + r[6] * q.z) / sc.x, (r[1] * q.x + r[4] * q.y + r[7] * q.z) / sc.y,
-                  (r[2] * q.x + r[5] * q.y + r[8] * q.z) / sc.z);
-        V3 B = v3((r[0] * d.x + r[3] * d.y + r[6] * d.z) / sc.x, (r[1] * d.x + r[4] * d.y + r[7] * d.z) / sc.y,
-                  (r[2] * d.x + r[5] * d.y + r[8] * d.z) / sc.z);
+        // ray in the stroke's unit space: A + t B = M (o + t d - tr), M = diag(1/s) R^T; the
+        // reciprocals are the hardware approximation (1 ulp), far inside the padding below
+        const float* M = rec + R_M;
+        V3 q = o - ld3(rec + R_TR);
+        V3 A = v3(M[0] * q.x + M[1] * q.y + M[2] * q.z, M[3] * q.x + M[4] * q.y + M[5] * q.z,
+                  M[6] * q.x + M[7] * q.y + M[8] * q.z);
+        V3 B = v3(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[3] * d.x + M[4] * d.y + M[5] * d.z,
+                  M[6] * d.x + M[7] * d.y + M[8] * d.z);
         float mu = isr ? m / rec[R_MINS] : m;
         float ex = rec[R_E0 + 0] + rec[R_E1 + 0] * mu;
         float ey = rec[R_E0 + 1] + rec[R_E1 + 1] * mu;
         float ez = rec[R_E0 + 2] + rec[R_E1 + 2] * mu;
-        slab(A.x, B.x, -ex, ex, t0, t1);
-        slab(A.y, B.y, -ey, ey, t0, t1);
-        slab(A.z, B.z, -ez, ez, t0, t1);
+        slab_inv(A.x, __builtin_amdgcn_rcpf(B.x), -ex, ex, t0, t1);
+        slab_inv(A.y, __builtin_amdgcn_rcpf(B.y), -ey, ey, t0, t1);
+        slab_inv(A.z, __builtin_amdgcn_rcpf(B.z), -ez, ez, t0, t1);
     }
     if (!(t0 <= t1)) return false;
     float pad = 1e-4f * (t1 - t0) + 1e-5f * (fabsf(t0) + fabsf(t1)) + 1e-6f;
@@ -303,6 +308,16 @@ __device__ __forceinline__ Pair eval_pair(const float* __restrict__ rec, int bas
     q.alpha = alpha_from_sdf(q.sdf, q.sc, A.delta, A.laplace != 0);
     return q;
 }
+
+// dL/dSDF of the exact adjoint as a function of the SDF value (alpha recomputed from it)
+struct ExactTransfer {
+    float ga, sc, delta;
+    bool laplace;
+    __device__ __forceinline__ float operator()(float sdf) const {
+        const float a = alpha_from_sdf(sdf, sc, delta, laplace);
+        return ga * dalpha_dsdf(sdf, a, sc, delta, laplace, true);
+    }
+};
 
 template <int CID>
 __device__ __forceinline__ V3 stroke_color_uniform(const float* __restrict__ cp, V3 vdir) {
@@ -996,12 +1011,22 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                             const bool curve = ibits(rec[R_CURVE]) != 0;
                             const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
                             const float* cp = A.cparams + (int64_t)k * A.Q;
+                            const float* prm = curve ? rec + R_CP : rec + R_BP;
+                            if (curve && A.exact) {
+                                // curves carry no transform: one closest-point solve gives the
+                                // value (hence alpha and dL/dSDF) and the parameter adjoint
+                                const ExactTransfer tr{ga, (A.laplace ? 2.f : 0.5f) / (A.delta * r), A.delta,
+                                                       A.laplace != 0};
+                                float sdf, gs;
+                                curve_value_grad(base, x, prm, tr, G, sdf, gs);
+                                color_grad_t<CID>(x, rc.vd, cp, gcol, G + 14, true);
+                                G[14 + NQ] += gdw;
+                            } else {
                             const Pair q = eval_pair(rec, base, curve, x, r, A);
                             V3 gpos = color_grad_t<CID>(q.p, rc.vd, cp, gcol, G + 14, A.exact != 0);
                             G[14 + NQ] += gdw;
                             const float gs = ga * dalpha_dsdf(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0,
                                                               A.exact != 0);
-                            const float* prm = curve ? rec + R_CP : rec + R_BP;
                             const V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
                             gpos = gpos + dsdp * gs;
                             if (!curve) {
@@ -1025,6 +1050,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                                 else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
                                 G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
                                 G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
+                            }
                             }
                         }
                     }

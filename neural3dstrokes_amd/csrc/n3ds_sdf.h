@@ -340,8 +340,7 @@ __device__ __forceinline__ V3 face_normal(V3 q, V3 p) {
 // quadratic Bezier: reverse-mode derivative of the closed-form solve itself
 // (cube roots / acos branch, including its 1e-8 regularisers), i.e. the exact
 // derivative of eval_quad_bezier.
-__device__ __forceinline__ V3 grad_ex_quad(V3 pos, const float* prm, float g, float* gp) {
-    QuadSolve Q = quad_solve(pos, prm);
+__device__ __forceinline__ V3 grad_ex_quad_solved(V3 pos, const float* prm, const QuadSolve& Q, float g, float* gp) {
     const float t = Q.t;
     const float r1 = prm[9], r2 = prm[10];
     const V3 v = Q.d + (Q.c + Q.b * t) * t;  // P(t) - pos
@@ -405,14 +404,18 @@ __device__ __forceinline__ V3 grad_ex_quad(V3 pos, const float* prm, float g, fl
     return -gd;
 }
 
+__device__ __forceinline__ V3 grad_ex_quad(V3 pos, const float* prm, float g, float* gp) {
+    const QuadSolve Q = quad_solve(pos, prm);
+    return grad_ex_quad_solved(pos, prm, Q, g, gp);
+}
+
 // cubic Bezier / Catmull-Rom: derivative through the polyline parameter.
 // Writes dSDF/dK (K = control points for Bezier, power coefficients for
 // Catmull-Rom) into gK and returns dSDF/dpos.
 template <bool POWER>
-__device__ __forceinline__ V3 grad_ex_curve_K(V3 pos, const V3 K[4], float r1, float r2, V3 gK[4], float& dr1,
-                                             float& dr2) {
+__device__ __forceinline__ V3 grad_ex_curve_picked(V3 pos, const V3 K[4], float r1, float r2, const CurvePick& P,
+                                                  V3 gK[4], float& dr1, float& dr2) {
     constexpr float step = 1.0f / CURVE_SEGMENTS;
-    CurvePick P = curve_pick<POWER>(pos, K);
     float t = P.t;
     V3 Pt = curve_point<POWER>(t, K);
     V3 v = Pt - pos;
@@ -443,6 +446,14 @@ __device__ __forceinline__ V3 grad_ex_curve_K(V3 pos, const V3 K[4], float r1, f
     dr2 = -t;
     return gpos;
 }
+
+template <bool POWER>
+__device__ __forceinline__ V3 grad_ex_curve_K(V3 pos, const V3 K[4], float r1, float r2, V3 gK[4], float& dr1,
+                                             float& dr2) {
+    const CurvePick P = curve_pick<POWER>(pos, K);
+    return grad_ex_curve_picked<POWER>(pos, K, r1, r2, P, gK, dr1, dr2);
+}
+
 
 // Catmull-Rom coefficient adjoint: dL/dK -> dL/d(p0..p3)
 __device__ __forceinline__ void catmull_coef_adjoint(const CatmullCoef& c, const V3 gK[4], V3 gp[4]) {
@@ -500,6 +511,44 @@ __device__ __forceinline__ void catmull_coef_adjoint(const CatmullCoef& c, const
     back(gt12, c.t12, c.L12, c.p1, c.p2, g1, g2);
     back(gt23, c.t23, c.L23, c.p2, c.p3, g2, g3);
     gp[0] = g0; gp[1] = g1; gp[2] = g2; gp[3] = g3;
+}
+
+// Exact SDF value of a curve base together with its adjoint from one
+// closest-point solve: dsdf(sdf) maps the value to dL/dSDF (the caller's
+// transfer derivative), the parameter gradients are accumulated into gp.
+template <typename DSDF>
+__device__ __forceinline__ V3 curve_value_grad(int base, V3 pos, const float* prm, DSDF dsdf, float* gp, float& sdf,
+                                               float& g) {
+    if (base == QUAD_BEZIER) {
+        const QuadSolve Q = quad_solve(pos, prm);
+        sdf = sqrtf(Q.dist2 + 1e-8f) - lerpf(prm[9], prm[10], Q.t);
+        g = dsdf(sdf);
+        return grad_ex_quad_solved(pos, prm, Q, g, gp);
+    }
+    V3 gK[4];
+    float dr1, dr2;
+    V3 gpos;
+    if (base == CUBIC_BEZIER) {
+        V3 K[4];
+        cubic_ctrl(prm, K);
+        const CurvePick P = curve_pick<false>(pos, K);
+        sdf = safe_len(curve_point<false>(P.t, K) - pos) - lerpf(prm[12], prm[13], P.t);
+        g = dsdf(sdf);
+        gpos = grad_ex_curve_picked<false>(pos, K, prm[12], prm[13], P, gK, dr1, dr2);
+    } else {
+        const CatmullCoef cc = catmull_coef(prm);
+        const CurvePick P = curve_pick<true>(pos, cc.K);
+        sdf = safe_len(curve_point<true>(P.t, cc.K) - pos) - lerpf(prm[12], prm[13], P.t);
+        g = dsdf(sdf);
+        V3 gK2[4];
+        gpos = grad_ex_curve_picked<true>(pos, cc.K, prm[12], prm[13], P, gK2, dr1, dr2);
+        catmull_coef_adjoint(cc, gK2, gK);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) add3(gp + 3 * j, gK[j] * g);
+    gp[12] += dr1 * g;
+    gp[13] += dr2 * g;
+    return gpos;
 }
 
 __device__ __forceinline__ V3 grad_ex(int base, V3 p, const float* prm, float g, float* gp) {

@@ -228,29 +228,28 @@ __device__ __forceinline__ void slab_inv(float o, float inv, float lo, float hi,
     t1 = fminf(t1, fmaxf(a, b));
 }
 
+__device__ __forceinline__ void pad_interval(float t0, float t1, float& ta, float& tb) {
+    float pad = 1e-4f * (t1 - t0) + 1e-5f * (fabsf(t0) + fabsf(t1)) + 1e-6f;
+    ta = t0 - pad;
+    tb = t1 + pad;
+}
+
+// Ray interval [ta, tb] inside the stroke's bound.  Solids: final.  Curves
+// (*curve = true): the whole-curve box only; the piece boxes refine it
+// (refine_curve_hits), one lane per piece.
 // di: componentwise 1 / d (curves are tested in world space, so it is per ray)
 __device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o, V3 d, V3 di, float m, int isr,
-                                            float tlo, float thi, float& ta, float& tb) {
+                                            float tlo, float thi, float& ta, float& tb, bool& curve) {
     float t0 = tlo, t1 = thi;
-    if (ibits(rec[R_CURVE])) {
+    curve = ibits(rec[R_CURVE]) != 0;
+    if (curve) {
         float e = rec[R_RMAX] + m;
-        float h0 = tlo, h1 = thi;
-        slab_inv(o.x, di.x, rec[R_HULL + 0] - e, rec[R_HULL + 3] + e, h0, h1);
-        slab_inv(o.y, di.y, rec[R_HULL + 1] - e, rec[R_HULL + 4] + e, h0, h1);
-        slab_inv(o.z, di.z, rec[R_HULL + 2] - e, rec[R_HULL + 5] + e, h0, h1);
-        if (!(h0 <= h1)) return false;
-        float u0 = 1e30f, u1 = -1e30f;
-#pragma unroll 4
-        for (int j = 0; j < NPIECE; j++) {
-            const float* bx = rec + R_BOX + 6 * j;
-            float p0 = h0, p1 = h1;
-            slab_inv(o.x, di.x, bx[0] - e, bx[3] + e, p0, p1);
-            slab_inv(o.y, di.y, bx[1] - e, bx[4] + e, p0, p1);
-            slab_inv(o.z, di.z, bx[2] - e, bx[5] + e, p0, p1);
-            if (p0 <= p1) { u0 = fminf(u0, p0); u1 = fmaxf(u1, p1); }
-        }
-        t0 = u0;
-        t1 = u1;
+        slab_inv(o.x, di.x, rec[R_HULL + 0] - e, rec[R_HULL + 3] + e, t0, t1);
+        slab_inv(o.y, di.y, rec[R_HULL + 1] - e, rec[R_HULL + 4] + e, t0, t1);
+        slab_inv(o.z, di.z, rec[R_HULL + 2] - e, rec[R_HULL + 5] + e, t0, t1);
+        ta = t0;
+        tb = t1;
+        return t0 <= t1;
     } else {
         // ray in the stroke's unit space: A + t B = M (o + t d - tr), M = diag(1/s) R^T; the
         // reciprocals are the hardware approximation (1 ulp), far inside the padding below
@@ -269,9 +268,7 @@ __device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o,
         slab_inv(A.z, __builtin_amdgcn_rcpf(B.z), -ez, ez, t0, t1);
     }
     if (!(t0 <= t1)) return false;
-    float pad = 1e-4f * (t1 - t0) + 1e-5f * (fabsf(t0) + fabsf(t1)) + 1e-6f;
-    ta = t0 - pad;
-    tb = t1 + pad;
+    pad_interval(t0, t1, ta, tb);
     return true;
 }
 
@@ -471,7 +468,53 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
                                            int base0, int lane, int nh, int np, int& tot) {
     const int k = A.S - 1 - (base0 + lane);
     float ta = 0.f, tb = -1.f;
-    bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi, ta, tb);
+    bool curve = false;
+    bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi,
+                                     ta, tb, curve);
+    // curves whose whole-curve box is hit: test their NPIECE piece boxes with one
+    // lane per piece (64 / NPIECE curves at a time), the union of the hit pieces
+    // is the interval (pb is free outside the evaluation windows)
+    static_assert(NPIECE == 16, "one 16-lane group per curve");
+    const unsigned long long cm = __ballot(hit && curve);
+    if (cm) {
+        int* src = (int*)w.pb;      // [64] lane of the c-th curve to refine
+        float* res = w.pb + 64;     // [64][2] refined interval
+        const int crank = __popcll(cm & ((1ull << lane) - 1ull));
+        if (hit && curve) src[crank] = lane;
+        wave_sync();
+        const int ncv = __popcll(cm);
+        for (int q0 = 0; q0 < ncv; q0 += 64 / NPIECE) {
+            const int cs = q0 + lane / NPIECE;
+            const int piece = lane % NPIECE;
+            float u0 = 1e30f, u1 = -1e30f;
+            if (cs < ncv) {
+                const float* rec = A.table + (int64_t)(A.S - 1 - (base0 + src[cs])) * REC;
+                const float e = rec[R_RMAX] + rc.m;
+                const float* bx = rec + R_BOX + 6 * piece;
+                float p0 = rc.tlo, p1 = rc.thi;
+                slab_inv(rc.oc.x, rc.di.x, bx[0] - e, bx[3] + e, p0, p1);
+                slab_inv(rc.oc.y, rc.di.y, bx[1] - e, bx[4] + e, p0, p1);
+                slab_inv(rc.oc.z, rc.di.z, bx[2] - e, bx[5] + e, p0, p1);
+                if (p0 <= p1) { u0 = p0; u1 = p1; }
+            }
+#pragma unroll
+            for (int off = NPIECE / 2; off > 0; off >>= 1) {
+                u0 = fminf(u0, __shfl_xor(u0, off, 64));
+                u1 = fmaxf(u1, __shfl_xor(u1, off, 64));
+            }
+            if (cs < ncv && piece == 0) {
+                res[2 * cs] = u0;
+                res[2 * cs + 1] = u1;
+            }
+        }
+        wave_sync();
+        if (hit && curve) {
+            const float u0 = res[2 * crank], u1 = res[2 * crank + 1];
+            hit = u0 <= u1;
+            if (hit) pad_interval(u0, u1, ta, tb);
+        }
+        wave_sync();
+    }
     int i0 = 0, cnt = 0;
     if (hit) {
         if (sorted) {

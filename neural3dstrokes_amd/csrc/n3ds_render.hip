@@ -393,13 +393,14 @@ struct WaveMem {
     int* ccnt;   // [HMAX] its number of pairs
     float* pb;   // [npf][PBUF] per-pair values of the current window
     float* scr;  // [64][ng+1] per-lane gradient slots (adjoint)
+    int* ul;     // [PBUF] window pairs with a nonzero adjoint (adjoint)
 };
 
 __host__ __device__ constexpr int wave_mem_floats(int ns, int npf, int ng) {
-    return 64 * ns + 4 * HMAX + npf * PBUF + (ng > 0 ? 64 * (ng + 1) : 0);
+    return 64 * ns + 4 * HMAX + npf * PBUF + (ng > 0 ? 64 * (ng + 1) + PBUF : 0);
 }
 
-__device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf) {
+__device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf, int ng = 0) {
     WaveMem w;
     w.ts = base;
     int* ib = (int*)(base + 64 * ns);
@@ -409,6 +410,7 @@ __device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf) {
     w.ccnt = ib + 3 * HMAX;
     w.pb = base + 64 * ns + 4 * HMAX;
     w.scr = w.pb + npf * PBUF;
+    w.ul = (int*)(w.scr + (ng > 0 ? 64 * (ng + 1) : 0));
     return w;
 }
 
@@ -801,7 +803,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
     float* dst = use_lds ? gacc : gdirect;
     for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) gacc[i] = 0.f;
     __syncthreads();
-    const WaveMem w = wave_mem(smem + nacc + wave * wave_mem_floats(NS, NPF, NG), NS, NPF);
+    const WaveMem w = wave_mem(smem + nacc + wave * wave_mem_floats(NS, NPF, NG), NS, NPF, NG);
     const int64_t N = A.R * (int64_t)A.n;
     const int64_t nwaves = (int64_t)gridDim.x * W;
     for (int64_t ray = (int64_t)blockIdx.x * W + wave; ray < A.R; ray += nwaves) {
@@ -885,7 +887,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
             suf_in += __shfl(v, 0, 64);
         }
         // per-sample compositing adjoint inputs
-        float gD[NS], O[NS], Tp[NS], D2[NS];
+        float gD[NS], O[NS], iOs[NS], Tp[NS], D2[NS];
         V3 gC[NS], C2[NS];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
@@ -893,6 +895,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
             const V3 gcv = grgb * wv[s];
             gC[s] = v3(inr[s][0] ? gcv.x : 0.f, inr[s][1] ? gcv.y : 0.f, inr[s][2] ? gcv.z : 0.f);
             O[s] = 1.f + 1e-6f - Tn[s];
+            iOs[s] = 1.f / O[s];
             Tp[s] = 1.f;
             D2[s] = 0.f;
             C2[s] = v3s(0.f);
@@ -1006,7 +1009,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                             const float sufD = Dt[s] - D2[s];
                             const V3 sufC = Ct[s] - C2[s];
                             const float inv1 = 1.f / fmaxf(1.f - a, 1e-6f);
-                            const float iO = 1.f / O[s];
+                            const float iO = iOs[s];
                             if (A.exact) {
                                 ga = gD[s] * (dk * Tk - sufD * inv1);
                                 ga += gC[s].x * ((c.x * Tk - sufC.x * inv1) * iO - Ct[s].x * (Tn[s] * inv1) * iO * iO);
@@ -1031,19 +1034,33 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                     }
                 }
                 wave_sync();
-                // (3) parameter gradients per pair, reduced per stroke
+                // (3) parameter gradients per pair, reduced per stroke.  Only pairs with a
+                //     nonzero adjoint contribute (alpha = 0 pairs do not): compact them first,
+                //     in order, so that each stroke's pairs stay contiguous
+                int nu = 0;
                 for (int p0 = 0; p0 < wp; p0 += 64) {
                     const int p = p0 + lane;
+                    const bool live = p < wp && (w.pb[p] != 0.f || w.pb[PBUF + p] != 0.f ||
+                                                 w.pb[2 * PBUF + p] != 0.f || w.pb[3 * PBUF + p] != 0.f ||
+                                                 w.pb[4 * PBUF + p] != 0.f);
+                    const unsigned long long lm = __ballot(live);
+                    if (live) w.ul[nu + __popcll(lm & ((1ull << lane) - 1ull))] = p;
+                    nu += __popcll(lm);
+                }
+                wave_sync();
+                for (int u0 = 0; u0 < nu; u0 += 64) {
+                    const int u = u0 + lane;
                     float G[NG];
 #pragma unroll
                     for (int g = 0; g < NG; g++) G[g] = 0.f;
-                    int c = c0;
-                    if (p < wp) {
+                    int c = 0x7fffffff;  // sorts after every hit index
+                    if (u < nu) {
+                        const int p = w.ul[u];
                         c = find_hit(w, c0, c1, wbase, p);
                         const float ga = w.pb[p];
                         const float gdw = w.pb[PBUF + p];
                         const V3 gcol = v3(w.pb[2 * PBUF + p], w.pb[3 * PBUF + p], w.pb[4 * PBUF + p]);
-                        if (ga != 0.f || gdw != 0.f || gcol.x != 0.f || gcol.y != 0.f || gcol.z != 0.f) {
+                        {
                             const int k = w.ck[c];
                             const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
                             V3 x;
@@ -1098,10 +1115,12 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                         }
                     }
                     // segmented sum over the lanes of each stroke through an LDS transpose
+                    // (column NG of a lane's row holds its hit index; sorted over the lanes)
 #pragma unroll
                     for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
+                    w.scr[lane * (NG + 1) + NG] = __int_as_float(c);
                     wave_sync();
-                    const int nl = min(64, wp - p0);
+                    const int nl = min(64, nu - u0);
                     const int cfirst = __builtin_amdgcn_readfirstlane(c);
                     const int clast = __builtin_amdgcn_readlane(c, nl - 1);
                     const int items = (clast - cfirst + 1) * NG;
@@ -1109,9 +1128,15 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
                         const int jseg = it / NG;
                         const int g = it - jseg * NG;
                         const int cc = cfirst + jseg;
-                        const int rel = w.coff[cc] - wbase - p0;
-                        const int lo = max(rel, 0);
-                        const int hi = min(rel + w.ccnt[cc], nl);
+                        // lanes [lo, hi) hold the pairs of hit cc
+                        int lo = 0, hi = 0;
+#pragma unroll
+                        for (int step = 64; step > 0; step >>= 1) {
+                            if (lo + step <= nl && __float_as_int(w.scr[(lo + step - 1) * (NG + 1) + NG]) < cc)
+                                lo += step;
+                            if (hi + step <= nl && __float_as_int(w.scr[(hi + step - 1) * (NG + 1) + NG]) <= cc)
+                                hi += step;
+                        }
                         float sum = 0.f;
                         for (int l = lo; l < hi; l++) sum += w.scr[l * (NG + 1) + g];
                         if (sum != 0.f) {

@@ -201,6 +201,39 @@ int n3ds_render_backward(const float* origins, const float* directions, const fl
                          int64_t workspace_floats, float* grad_shape, float* grad_color, float* grad_density,
                          void* stream);
 
+/* ------------------------------------------------------------------------
+ * Stroke-parameter update (one training step's optimizer stage).
+ * Together these replace, for fp32 CUDA parameters, the reference's
+ * clip_gradients (source/utils/training.py:67-77), torch.optim.Adam as built
+ * by create_optimizer (training.py:119-136; betas / eps from
+ * source/configs.py:82-84) and StrokeField.clip_params
+ * (source/models.py:576-584).
+ * ------------------------------------------------------------------------ */
+
+/* *acc (device double, zero before the first tensor of a step) += sum of
+ * grads[i]^2 over n floats (the global norm of clip_grad_norm_ is accumulated
+ * tensor by tensor). */
+int n3ds_grad_sqnorm_accumulate(const float* grads, int64_t n, double* acc, void* stream);
+
+/* *coef = min(max_norm / (sqrt(*acc) + 1e-6), 1), NaN propagating
+ * (torch.nn.utils.clip_grad_norm_), then *acc = 0 for the next step. */
+int n3ds_grad_clip_coef(double* acc, float max_norm, float* coef, void* stream);
+
+/* In place, for one parameter tensor [n_rows, n_cols]:
+ *   g *= *grad_coef (if grad_coef != NULL); g = clamp(g, -v, v) if
+ *   grad_max_val = v > 0; g = nan_to_num(g) (written back to grads);
+ *   Adam (torch.optim.Adam semantics, step = t after increment, t >= 1).
+ *   When step_dev / lr_dev (device floats) are given, t / lr are read from
+ *   them, so a captured step replays with the current count and rate
+ *   (the per-step rate of learning_rate_decay, training.py:89-116);
+ *   p = clamp(p, col_lo[c], col_hi[c]) per column when col_lo / col_hi
+ *   (device [n_cols], +-inf for an open side) are given, then
+ *   p = clamp(p, lo_all, hi_all).  n_cols <= 1024. */
+int n3ds_adam_step(float* params, float* grads, float* exp_avg, float* exp_avg_sq, int64_t n_rows, int n_cols,
+                   const float* col_lo, const float* col_hi, double lr, const float* lr_dev, double beta1,
+                   double beta2, double eps, int64_t step, const float* step_dev, float grad_max_val,
+                   const float* grad_coef, float lo_all, float hi_all, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ _I64 = ctypes.c_int64
 _I32 = ctypes.c_int
 _U32 = ctypes.c_uint32
 _F = ctypes.c_float
+_D = ctypes.c_double
 
 SIGNATURES = {
     'n3ds_version': (_I32, []),
@@ -34,6 +35,9 @@ SIGNATURES = {
                                          _I32, _I32, _P, _I64, _P]),
     'n3ds_grid_encode_backward_workspace_bytes': (_I64, [_I64, _I32, _I32, _I32, _I64]),
     'n3ds_grid_grad_tv': (_I32, [_P, _P, _P, _P, _F, _I64, _I32, _I32, _I32, _F, _I32, _I32, _I32, _I32, _P]),
+    'n3ds_grad_sqnorm_accumulate': (_I32, [_P, _I64, _P, _P]),
+    'n3ds_grad_clip_coef': (_I32, [_P, _F, _P, _P]),
+    'n3ds_adam_step': (_I32, [_P, _P, _P, _P, _I64, _I32, _P, _P, _D, _P, _D, _D, _D, _I64, _P, _F, _P, _F, _F, _P]),
     'n3ds_render_table_floats': (_I64, [_I32]),
     'n3ds_render_prepare': (_I32, [_P, _P, _P, _I32, _I32, _P, _P]),
     'n3ds_render_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,

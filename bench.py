@@ -282,6 +282,70 @@ def memory_block(steps=3, warmup=1):
     return out
 
 
+def multiscene_block(n_scenes=32, steps=10, warmup=2, seed=0, loop_steps=3):
+    """32 independent scenes (strokes log-U[20,1000], rays log-U[1024,65536], 128 samples):
+    one photometric training step (forward, loss, backward, Adam lr 1e-3) for all scenes as
+    one replayed HIP graph with one launch per stage, against a per-scene launch loop
+    (each scene rendered, differentiated and updated on its own, eagerly)."""
+    import torch
+    from neural3dstrokes_amd import optim, render, scenes
+    from neural3dstrokes_amd.multiscene import MultiSceneTrainer, SceneBatch, pack
+    dev = torch.device('cuda:0')
+    sc, rc, kc = scenes.make_multiscene(n_scenes=n_scenes, seed=seed)
+    n = sc[0]['tdist'].shape[1] - 1
+    packed = {k: v.to(dev) for k, v in pack(sc).items()}
+    batch = SceneBatch(rc, kc, n, packed['shape_params'].shape[1], 3, 0, device=dev)
+    g = torch.Generator().manual_seed(seed + 1)
+    targets = torch.rand(batch.n_rays, 3, generator=g).to(dev)
+    trainer = MultiSceneTrainer(batch, packed, packed['stroke_ids'], packed['shape_params'], packed['color_params'],
+                                packed['density_params'], targets, lr=1e-3)
+    trainer.capture(warmup=warmup)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    trainer.step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        trainer.step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1) / steps
+    # per-scene launch loop: own leaves and optimizer per scene
+    keys = ('origins', 'directions', 'viewdirs', 'radii', 'tdist')
+    per = []
+    for i in range(batch.n_scenes):
+        rs, ks = batch.rays(i), batch.strokes(i)
+        lv = [packed[k][ks].clone().requires_grad_(True) for k in ('shape_params', 'color_params', 'density_params')]
+        rays = {k: packed[k][rs].contiguous() for k in keys}
+        opt = optim.Adam([dict(params=[lv[0]]), dict(params=[lv[1]]), dict(params=[lv[2]], min_value=0.0)], lr=1e-3)
+        per.append((rays, packed['stroke_ids'][ks].contiguous(), lv, targets[rs], opt))
+
+    def loop_step():
+        for rays, ids, lv, tg, opt in per:
+            rgb, _, _ = render.render_strokes(rays['origins'], rays['directions'], rays['viewdirs'], rays['radii'],
+                                              rays['tdist'], ids, *lv)
+            ((rgb - tg) ** 2).mean(0).sum().backward()
+            opt.step()
+            for p in lv:
+                p.grad = None
+
+    for _ in range(warmup):
+        loop_step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(loop_steps):
+        loop_step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_loop = e0.elapsed_time(e1) / loop_steps
+    rs_total = batch.n_rays * n
+    return {'workload': f'{n_scenes} scenes, strokes log-U[20,1000] (sum {batch.n_strokes}, max {batch.s_max}), '
+                        f'rays log-U[1024,65536] (sum {batch.n_rays}) x {n} samples, Adam lr 1e-3',
+            'graph_step_ms': ms_graph, 'graph_ray_samples_per_s': rs_total / (ms_graph * 1e-3),
+            'per_scene_loop_ms': ms_loop, 'per_scene_loop_ray_samples_per_s': rs_total / (ms_loop * 1e-3),
+            'speedup_vs_loop': ms_loop / ms_graph, 'backward_blocks': batch.n_blocks,
+            'loss': float(trainer.loss)}
+
+
 def cpu_baseline(sc, n_rays, g_rgb, g_acc):
     """Float32 CPU oracle (plain per-op math, no culling) fwd+bwd on the first n_rays rays."""
     import torch
@@ -320,6 +384,7 @@ def main():
     ap.add_argument('--fd-params', type=int, default=32, help='parameters for the finite-difference check (0: skip)')
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
+    ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build measures one GPU; multi-GPU runs are not part of it',
@@ -448,6 +513,8 @@ def main():
         out['hash_grid'] = grid_block()
     if args.memory:
         out['memory_bounded'] = memory_block()
+    if args.multiscene:
+        out['multiscene'] = multiscene_block()
     if args.cpu_rays > 0:
         out['cpu_baseline'] = cpu_baseline(sc, min(args.cpu_rays, R), g_rgb, g_acc)
     print(json.dumps(out))

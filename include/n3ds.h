@@ -202,6 +202,64 @@ int n3ds_render_backward(const float* origins, const float* directions, const fl
                          void* stream);
 
 /* ------------------------------------------------------------------------
+ * Batches of independent scenes.  No counterpart in the reference, which
+ * trains one scene per process (train.py:127-144): many scenes with uneven
+ * stroke and ray counts render in one launch per stage.  Rays and strokes are
+ * packed scene after scene in the same per-ray / per-stroke arrays as above
+ * (the stroke table is n3ds_render_prepare over all packed strokes).  The
+ * scene table is int32: [n_scenes, 8] records {first ray, rays, first stroke,
+ * strokes, first backward block, backward blocks, offset of the scene's
+ * partial gradient rows, backward wavefronts per block (0: accumulators in
+ * global memory)} followed by the scene of every backward block.  It is passed
+ * twice, in host memory (validated and used to size launches) and as a device
+ * copy (read by the kernels).  Results per scene equal n3ds_render_forward /
+ * _backward on that scene alone.
+ * ------------------------------------------------------------------------ */
+
+/* Fills record fields 4-7 (the backward plan: blocks per scene in proportion
+ * to each scene's rays x strokes, oversubscribing the device so the hardware
+ * dispatcher balances uneven scenes) from fields 0-3, and the block map when
+ * block_map holds block_map_cap >= the block count; returns the number of
+ * backward blocks, or the negated status on error. */
+int n3ds_render_batch_plan(int32_t* scenes_host, int n_scenes, int n_samples, int shape_width, int color_width,
+                           int color_id, int32_t* block_map, int block_map_cap);
+
+/* Forward of every scene of the batch: as n3ds_render_forward with
+ * n_strokes = all packed strokes; the kept-hit layout is sized by the
+ * largest scene's strokes. */
+int n3ds_render_batch_forward(const float* origins, const float* directions, const float* viewdirs,
+                              const float* radii, const float* tdist, const float* table, const float* color_params,
+                              const float* density_params, const int32_t* scenes_host, const int32_t* scenes,
+                              int n_scenes, int64_t n_rays, int n_samples, int n_strokes, int shape_width,
+                              int color_width, int color_id, float sdf_delta, int use_laplace_transform,
+                              int inv_scale_radius, int cone, float coord_div, const float* bg_color, float cull_eps,
+                              float* rgb_out, float* acc_out, float* depth_out, float* state_out,
+                              unsigned long long* counters, int* hit_dir, int* hit_buf, float* pair_buf, int cap_hits,
+                              int cap_pairs, void* stream);
+
+/* Floats of device workspace n3ds_render_batch_backward needs for a planned
+ * scene table (-1 on a HIP error). */
+int64_t n3ds_render_batch_backward_workspace_floats(const int32_t* scenes_host, int n_scenes, int n_samples,
+                                                    int shape_width, int color_width, int color_id);
+
+/* Backward of every scene of the batch: one launch per class of scenes (scenes
+ * whose LDS accumulators allow the same wavefronts per block; usually one or
+ * two classes), each block working on one scene, plus one reduction;
+ * accumulates into the packed gradient tensors.  The scene table must be
+ * planned (n3ds_render_batch_plan, block map included in both copies). */
+int n3ds_render_batch_backward(const float* origins, const float* directions, const float* viewdirs,
+                               const float* radii, const float* tdist, const float* table, const float* color_params,
+                               const float* density_params, const int32_t* scenes_host, const int32_t* scenes,
+                               int n_scenes, int64_t n_rays, int n_samples, int n_strokes, int shape_width,
+                               int color_width, int color_id, float sdf_delta, int use_laplace_transform,
+                               int inv_scale_radius, int cone, float coord_div, const float* bg_color, float cull_eps,
+                               int adjoint_exact, const float* grad_rgb, const float* grad_acc,
+                               const float* grad_depth, const float* state, const int* hit_dir, const int* hit_buf,
+                               const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
+                               int64_t workspace_floats, float* grad_shape, float* grad_color, float* grad_density,
+                               void* stream);
+
+/* ------------------------------------------------------------------------
  * Stroke-parameter update (one training step's optimizer stage).
  * Together these replace, for fp32 CUDA parameters, the reference's
  * clip_gradients (source/utils/training.py:67-77), torch.optim.Adam as built

@@ -46,6 +46,14 @@ SIGNATURES = {
     'n3ds_render_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
                                     _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P,
                                     _P, _P]),
+    'n3ds_render_batch_plan': (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32]),
+    'n3ds_render_batch_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32, _I32,
+                                         _I32, _F, _I32, _I32, _I32, _F, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32,
+                                         _I32, _P]),
+    'n3ds_render_batch_backward_workspace_floats': (_I64, [_P, _I32, _I32, _I32, _I32, _I32]),
+    'n3ds_render_batch_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32, _I32,
+                                          _I32, _F, _I32, _I32, _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P,
+                                          _I32, _I32, _P, _I64, _P, _P, _P, _P]),
 }
 
 _lib = None

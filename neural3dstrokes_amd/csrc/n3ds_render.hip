@@ -87,7 +87,40 @@ struct RenderArgs {
     int* hbuf;
     float* pbuf;
     int cap_h, cap_p, nch;
+    int nch_stride;  // chunks per ray in hdir (the largest scene's in a batch)
+    // batch of independent scenes (nullptr: one scene).  Per scene SCENE_REC ints
+    // {first ray, rays, first stroke, strokes, first backward block, backward blocks,
+    // offset of its partial gradient rows, backward wavefronts per block (0: global
+    // accumulators)}, then the scene of every backward block; rays and strokes of one
+    // scene are contiguous in every per-ray / per-stroke array.
+    const int* scenes;
+    int n_scenes;
+    int block_base;  // backward: global block number of this launch's first block
 };
+
+constexpr int SCENE_REC = 8;
+
+// last scene whose field f is <= v (scenes with nothing in that field share their
+// start with the next scene, so the search lands on the one that owns v)
+__device__ __forceinline__ int find_scene(const int* sc, int n_scenes, int f, int v) {
+    int lo = 0, hi = n_scenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sc[mid * SCENE_REC + f] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// point the stroke arrays of A at one scene of a batch
+__device__ __forceinline__ void scene_strokes(RenderArgs& A, const int* rec) {
+    const int sb = rec[2], ns = rec[3];
+    A.table += (int64_t)sb * REC;
+    A.cparams += (int64_t)sb * A.Q;
+    A.dparams += sb;
+    A.S = ns;
+    A.nch = (ns + 63) / 64;
+}
 
 __device__ __forceinline__ int ibits(float f) { return __float_as_int(f); }
 __device__ __forceinline__ float fbits(int i) { return __int_as_float(i); }
@@ -552,7 +585,7 @@ __device__ __forceinline__ int window_end(const WaveMem& w, int c0, int nh, int 
 // forward
 // ---------------------------------------------------------------------------
 template <int NS, int CID>
-__global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, float* __restrict__ rgb_out,
+__global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, float* __restrict__ rgb_out,
                                                                  float* __restrict__ acc_out,
                                                                  float* __restrict__ depth_out,
                                                                  float* __restrict__ state,
@@ -561,7 +594,12 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
     constexpr int NPF = CID == 1 ? 4 : 1;  // alpha (+ colour for gradient_rgb)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t ray = (int64_t)blockIdx.x * FWAVES + wave;
-    if (ray >= A.R) return;
+    if (ray >= A0.R) return;
+    RenderArgs A = A0;
+    if (A0.scenes) {
+        const int sc = find_scene(A0.scenes, A0.n_scenes, 0, __builtin_amdgcn_readfirstlane((int)ray));
+        scene_strokes(A, A0.scenes + sc * SCENE_REC);
+    }
     const WaveMem w = wave_mem(smem + wave * wave_mem_floats(NS, NPF, 0), NS, NPF);
     const RayConst rc = load_ray(A, ray);
     float ts[NS];
@@ -588,7 +626,7 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A, f
             const bool keep = !full && hoff + nc <= A.cap_h && poff + tot <= A.cap_p;
             full = !keep;
             if (lane == 0) {
-                int* e = A.hdir + (ray * A.nch + ch) * 3;
+                int* e = A.hdir + (ray * A.nch_stride + ch) * 3;
                 e[0] = keep ? hoff : -1;
                 e[1] = poff;
                 e[2] = nc;
@@ -785,7 +823,7 @@ __device__ __forceinline__ int slot_column(int slot, const float* __restrict__ r
 }
 
 template <int NS, int CID>
-__global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, const float* __restrict__ g_rgb,
+__global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, const float* __restrict__ g_rgb,
                                                                  const float* __restrict__ g_acc,
                                                                  const float* __restrict__ g_depth,
                                                                  const float* __restrict__ state,
@@ -797,16 +835,32 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
     constexpr int NPF = 5;       // per-pair values: alpha -> dL/dalpha, colour / dL/d(density, colour)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = blockDim.x >> 6;
-    const int PG = A.Pmax + A.Q + 1;
+    const int PG = A0.Pmax + A0.Q + 1;
+    // LDS / partial rows are sized for the largest scene; this block's scene may be smaller
+    const int64_t nacc_max = use_lds ? (int64_t)A0.S * PG : 0;
+    RenderArgs A = A0;
+    int64_t ray0 = (int64_t)blockIdx.x * W + wave, ray_end = A0.R, ray_step = (int64_t)gridDim.x * W;
+    float* dst = use_lds ? smem : gdirect;
+    float* prow = partials ? partials + (int64_t)blockIdx.x * nacc_max : nullptr;
+    if (A0.scenes) {  // a batch: this block works on one scene's rays
+        const int b = A0.block_base + (int)blockIdx.x;
+        const int sc = A0.scenes[A0.n_scenes * SCENE_REC + b];
+        const int* rec = A0.scenes + sc * SCENE_REC;
+        scene_strokes(A, rec);
+        const int j = b - rec[4];
+        ray0 = rec[0] + (int64_t)j * W + wave;
+        ray_end = (int64_t)rec[0] + rec[1];
+        ray_step = (int64_t)rec[5] * W;
+        if (!use_lds) dst = gdirect + (int64_t)rec[2] * PG;
+        if (partials) prow = partials + rec[6] + (int64_t)j * rec[3] * PG;
+    }
     const int64_t nacc = use_lds ? (int64_t)A.S * PG : 0;
     float* gacc = smem;
-    float* dst = use_lds ? gacc : gdirect;
     for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) gacc[i] = 0.f;
     __syncthreads();
-    const WaveMem w = wave_mem(smem + nacc + wave * wave_mem_floats(NS, NPF, NG), NS, NPF, NG);
+    const WaveMem w = wave_mem(smem + nacc_max + wave * wave_mem_floats(NS, NPF, NG), NS, NPF, NG);
     const int64_t N = A.R * (int64_t)A.n;
-    const int64_t nwaves = (int64_t)gridDim.x * W;
-    for (int64_t ray = (int64_t)blockIdx.x * W + wave; ray < A.R; ray += nwaves) {
+    for (int64_t ray = ray0; ray < ray_end; ray += ray_step) {
         const RayConst rc = load_ray(A, ray);
         float ts[NS];
         bool valid[NS];
@@ -909,7 +963,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
         for (int ch = 0; ch < A.nch;) {
             int hoff = -1, kpoff = 0, knc = 0;
             if (A.hdir) {
-                const int* e = A.hdir + (ray * A.nch + ch) * 3;
+                const int* e = A.hdir + (ray * A.nch_stride + ch) * 3;
                 hoff = __builtin_amdgcn_readfirstlane(e[0]);
                 kpoff = __builtin_amdgcn_readfirstlane(e[1]);
                 knc = __builtin_amdgcn_readfirstlane(e[2]);
@@ -1154,8 +1208,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A, c
     }
     if (use_lds) {
         __syncthreads();
-        float* out = partials + (int64_t)blockIdx.x * nacc;
-        for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) out[i] = gacc[i];
+        for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) prow[i] = gacc[i];
     }
 }
 
@@ -1169,6 +1222,31 @@ __global__ void render_reduce_kernel(const float* __restrict__ partials, int npa
     float s = 0.f;
     for (int b = 0; b < nparts; b++) s += partials[(int64_t)b * S * PG + i];
     int k = (int)(i / PG), c = (int)(i - (int64_t)k * PG);
+    if (c < Pmax) gshape[(int64_t)k * Pmax + c] += s;
+    else if (c < Pmax + Q) gcolor[(int64_t)k * Q + (c - Pmax)] += s;
+    else gdens[k] += s;
+}
+
+// batch of scenes: stroke k of scene s sums the partial rows of s's blocks
+// (rows of s's S * PG floats from the scene's offset), or, for a scene whose
+// accumulators did not fit in LDS, reads its row of the global accumulators
+__global__ void render_batch_reduce_kernel(const float* __restrict__ partials, const float* __restrict__ direct,
+                                           const int* __restrict__ scenes, int n_scenes, int S_total, int Pmax,
+                                           int Q, float* __restrict__ gshape, float* __restrict__ gcolor,
+                                           float* __restrict__ gdens) {
+    const int PG = Pmax + Q + 1;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)S_total * PG) return;
+    const int k = (int)(i / PG), c = (int)(i - (int64_t)k * PG);
+    const int* rec = scenes + find_scene(scenes, n_scenes, 2, k) * SCENE_REC;
+    float s = 0.f;
+    if (rec[7] == 0) {
+        if (rec[5] > 0) s = direct[i];
+    } else {
+        const int64_t row = (int64_t)rec[3] * PG;
+        const float* p = partials + rec[6] + (int64_t)(k - rec[2]) * PG + c;
+        for (int b = 0; b < rec[5]; b++) s += p[b * row];
+    }
     if (c < Pmax) gshape[(int64_t)k * Pmax + c] += s;
     else if (c < Pmax + Q) gcolor[(int64_t)k * Q + (c - Pmax)] += s;
     else gdens[k] += s;
@@ -1281,6 +1359,10 @@ static RenderArgs make_args(const float* origins, const float* dirs, const float
     A.pbuf = nullptr;
     A.cap_h = A.cap_p = 0;
     A.nch = (S + 63) / 64;
+    A.nch_stride = A.nch;
+    A.scenes = nullptr;
+    A.n_scenes = 0;
+    A.block_base = 0;
     return A;
 }
 
@@ -1406,6 +1488,279 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
     N3DS_LAUNCH_CHECK();
     hipLaunchKernelGGL(render_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace,
                        bs.use_lds ? (int)nb : 1, n_strokes, shape_width, color_width, grad_shape, grad_color,
+                       grad_density);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches of independent scenes (one launch per stage for all of them)
+// ---------------------------------------------------------------------------
+
+// validate a host scene table (fields 0-3) against the packed sizes; returns the largest scene's strokes
+static int batch_check(const int32_t* sc, int n_scenes, int64_t n_rays, int n_strokes, int* s_max) {
+    N3DS_REQUIRE(sc && n_scenes >= 1, "a batch needs a scene table with at least one scene");
+    N3DS_REQUIRE(n_rays < (int64_t)1 << 31 && n_strokes >= 0, "batched rays / strokes must fit int32");
+    int64_t r = 0, k = 0;
+    int m = 0;
+    for (int i = 0; i < n_scenes; i++) {
+        const int32_t* e = sc + (int64_t)i * SCENE_REC;
+        N3DS_REQUIRE(e[0] == r && e[1] >= 0 && e[2] == k && e[3] >= 0,
+                     "scene %d: rays / strokes must be packed in scene order (got first ray %d, first stroke %d)", i,
+                     e[0], e[2]);
+        r += e[1];
+        k += e[3];
+        if (e[3] > m) m = e[3];
+    }
+    N3DS_REQUIRE(r == n_rays && k == n_strokes, "scene table covers %lld rays / %lld strokes, arrays hold %lld / %d",
+                 (long long)r, (long long)k, (long long)n_rays, n_strokes);
+    *s_max = m;
+    return N3DS_OK;
+}
+
+// persistent backward blocks that fit on the device at once for this launch shape
+static int bwd_resident_blocks(int ns, int cid, const BwdShape& bs, int64_t* out) {
+    BwdKernel kern = bwd_kernel_ptr(ns, cid);
+    if (bs.lds_bytes > 65536)
+        N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bs.lds_bytes));
+    int per_cu = 0;
+    N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * bs.waves, (size_t)bs.lds_bytes));
+    if (per_cu < 1) per_cu = 1;
+    int64_t nb = (int64_t)g_num_cu * per_cu;
+    if (nb > (int64_t)g_num_cu * 8) nb = (int64_t)g_num_cu * 8;
+    *out = nb;
+    return N3DS_OK;
+}
+
+// Backward blocks per scene, beyond what is resident at once, so the hardware
+// dispatcher evens out scenes and rays of uneven cost; each block gets at least
+// BATCH_MIN_RAYS rays per wavefront.
+constexpr int BATCH_OVERSUB = 8;
+constexpr int BATCH_MIN_RAYS = 4;
+
+N3DS_EXPORT int n3ds_render_batch_plan(int32_t* scenes, int n_scenes, int n_samples, int shape_width,
+                                       int color_width, int color_id, int32_t* block_map, int block_map_cap) {
+    int64_t R = 0;
+    int K = 0;
+    for (int i = 0; i < n_scenes && scenes; i++) {
+        R += scenes[(int64_t)i * SCENE_REC + 1];
+        K += scenes[(int64_t)i * SCENE_REC + 3];
+    }
+    int s_max = 0;
+    int st = batch_check(scenes, n_scenes, R, K, &s_max);
+    if (st) return -st;
+    st = render_check(R, n_samples, s_max, shape_width, color_width, color_id);
+    if (st) return -st;
+    st = device_info();
+    if (st) return -st;
+    const int ns = (n_samples + 63) / 64, PG = shape_width + color_width + 1;
+    // launch class of each scene: wavefronts per block that fit its LDS accumulators
+    // (0: accumulators in global memory); one backward launch per class
+    const int classes[4] = {8, 4, 2, 0};
+    double work[4] = {0, 0, 0, 0}, tot = 0.0;
+    int64_t resident[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n_scenes; i++) {
+        int32_t* e = scenes + (int64_t)i * SCENE_REC;
+        const BwdShape bs = bwd_shape(ns, color_id, e[3], PG);
+        e[7] = bs.use_lds ? bs.waves : 0;
+        if (e[1] > 0 && e[3] > 0) {
+            const double w = (double)e[1] * (e[3] + 32.0);
+            tot += w;
+            for (int c = 0; c < 4; c++)
+                if (classes[c] == e[7]) work[c] += w;
+        }
+    }
+    for (int c = 0; c < 4; c++) {
+        if (work[c] == 0.0) continue;
+        int sm = 0;
+        for (int i = 0; i < n_scenes; i++)
+            if (scenes[(int64_t)i * SCENE_REC + 7] == classes[c] && scenes[(int64_t)i * SCENE_REC + 3] > sm)
+                sm = scenes[(int64_t)i * SCENE_REC + 3];
+        st = bwd_resident_blocks(ns, color_id, bwd_shape(ns, color_id, sm, PG), &resident[c]);
+        if (st) return -st;
+    }
+    // blocks numbered class by class, scene by scene within a class; partial rows packed likewise
+    int64_t b = 0, poff = 0;
+    for (int c = 0; c < 4; c++) {
+        for (int i = 0; i < n_scenes; i++) {
+            int32_t* e = scenes + (int64_t)i * SCENE_REC;
+            if (e[7] != classes[c]) continue;
+            int64_t g = 0;
+            if (e[1] > 0 && e[3] > 0) {
+                const double w = (double)e[1] * (e[3] + 32.0);
+                const int waves = e[7] ? e[7] : BWAVES;
+                g = (int64_t)((double)BATCH_OVERSUB * resident[c] * w / tot + 0.5);
+                const int64_t cap = (e[1] + (int64_t)waves * BATCH_MIN_RAYS - 1) / ((int64_t)waves * BATCH_MIN_RAYS);
+                if (g > cap) g = cap;
+                if (g < 1) g = 1;
+            }
+            e[4] = (int32_t)b;
+            e[5] = (int32_t)g;
+            e[6] = (int32_t)poff;
+            if (e[7] > 0) poff += g * e[3] * PG;
+            if (block_map && b + g <= block_map_cap)
+                for (int64_t k = 0; k < g; k++) block_map[b + k] = i;
+            b += g;
+            N3DS_REQUIRE(b < (1 << 30) && poff < ((int64_t)1 << 31) - 1, "batch too large for the backward plan");
+        }
+    }
+    return (int)b;
+}
+
+N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* directions, const float* viewdirs,
+                                          const float* radii, const float* tdist, const float* table,
+                                          const float* color_params, const float* density_params,
+                                          const int32_t* scenes_host, const int32_t* scenes, int n_scenes,
+                                          int64_t n_rays, int n_samples, int n_strokes, int shape_width,
+                                          int color_width, int color_id, float sdf_delta, int use_laplace_transform,
+                                          int inv_scale_radius, int cone, float coord_div, const float* bg_color,
+                                          float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
+                                          float* state_out, unsigned long long* counters, int* hit_dir,
+                                          int* hit_buf, float* pair_buf, int cap_hits, int cap_pairs,
+                                          void* stream) {
+    int s_max = 0;
+    int st = batch_check(scenes_host, n_scenes, n_rays, n_strokes, &s_max);
+    if (st) return st;
+    N3DS_REQUIRE(scenes, "the device scene table is missing");
+    st = render_check(n_rays, n_samples, s_max, shape_width, color_width, color_id);
+    if (st) return st;
+    if (n_rays == 0) return N3DS_OK;
+    RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
+                             n_rays, n_samples, s_max, shape_width, color_width, color_id, sdf_delta,
+                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps, 0);
+    A.scenes = scenes;
+    A.n_scenes = n_scenes;
+    st = set_keep(A, hit_dir, hit_buf, pair_buf, cap_hits, cap_pairs);
+    if (st) return st;
+    dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
+    const int ns = (n_samples + 63) / 64;
+    hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
+                       A, rgb_out, acc_out, depth_out, state_out, counters);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+// workspace of a planned batch: packed partial rows, then global accumulators for
+// all strokes if any scene's accumulators do not fit in LDS
+static void batch_workspace(const int32_t* sc, int n_scenes, int PG, int64_t* parts, int64_t* direct) {
+    int64_t p = 0, K = 0;
+    bool any_direct = false;
+    for (int i = 0; i < n_scenes; i++) {
+        const int32_t* e = sc + (int64_t)i * SCENE_REC;
+        K += e[3];
+        if (e[7] > 0) {
+            const int64_t end = e[6] + (int64_t)e[5] * e[3] * PG;
+            if (end > p) p = end;
+        } else if (e[5] > 0) {
+            any_direct = true;
+        }
+    }
+    *parts = p;
+    *direct = any_direct ? K * PG : 0;
+}
+
+N3DS_EXPORT int64_t n3ds_render_batch_backward_workspace_floats(const int32_t* scenes_host, int n_scenes,
+                                                                int n_samples, int shape_width, int color_width,
+                                                                int color_id) {
+    (void)n_samples;
+    (void)color_id;
+    if (!scenes_host || n_scenes < 1) return -1;
+    int64_t p, d;
+    batch_workspace(scenes_host, n_scenes, shape_width + color_width + 1, &p, &d);
+    return p + d;
+}
+
+N3DS_EXPORT int n3ds_render_batch_backward(const float* origins, const float* directions, const float* viewdirs,
+                                           const float* radii, const float* tdist, const float* table,
+                                           const float* color_params, const float* density_params,
+                                           const int32_t* scenes_host, const int32_t* scenes, int n_scenes,
+                                           int64_t n_rays, int n_samples, int n_strokes, int shape_width,
+                                           int color_width, int color_id, float sdf_delta, int use_laplace_transform,
+                                           int inv_scale_radius, int cone, float coord_div, const float* bg_color,
+                                           float cull_eps, int adjoint_exact, const float* grad_rgb,
+                                           const float* grad_acc, const float* grad_depth, const float* state,
+                                           const int* hit_dir, const int* hit_buf, const float* pair_buf,
+                                           int cap_hits, int cap_pairs, float* workspace, int64_t workspace_floats,
+                                           float* grad_shape, float* grad_color, float* grad_density, void* stream) {
+    int s_max = 0;
+    int st = batch_check(scenes_host, n_scenes, n_rays, n_strokes, &s_max);
+    if (st) return st;
+    N3DS_REQUIRE(scenes, "the device scene table is missing");
+    st = render_check(n_rays, n_samples, s_max, shape_width, color_width, color_id);
+    if (st) return st;
+    st = device_info();
+    if (st) return st;
+    const int PG = shape_width + color_width + 1;
+    const int ns = (n_samples + 63) / 64;
+    int64_t nb = 0;
+    for (int i = 0; i < n_scenes; i++) {
+        const int32_t* e = scenes_host + (int64_t)i * SCENE_REC;
+        const BwdShape bs = bwd_shape(ns, color_id, e[3], PG);
+        N3DS_REQUIRE(e[5] >= 0 && (e[5] > 0 || e[1] == 0 || e[3] == 0) && e[5] <= e[1] &&
+                         e[7] == (bs.use_lds ? bs.waves : 0),
+                     "scene %d has no valid backward plan (run n3ds_render_batch_plan)", i);
+        nb += e[5];
+    }
+    // the block map (after the scene records, same in the host and device copies)
+    const int32_t* map = scenes_host + (int64_t)n_scenes * SCENE_REC;
+    for (int i = 0; i < n_scenes; i++) {
+        const int32_t* e = scenes_host + (int64_t)i * SCENE_REC;
+        N3DS_REQUIRE(e[4] >= 0 && e[4] + (int64_t)e[5] <= nb, "scene %d: backward blocks out of range", i);
+        for (int k = 0; k < e[5]; k++)
+            N3DS_REQUIRE(map[e[4] + k] == i, "block map entry %d is not scene %d", e[4] + k, i);
+    }
+    if (n_rays == 0 || n_strokes == 0 || nb == 0) return N3DS_OK;
+    int64_t wp, wd;
+    batch_workspace(scenes_host, n_scenes, PG, &wp, &wd);
+    N3DS_REQUIRE(workspace_floats >= wp + wd, "workspace has %lld floats, need %lld", (long long)workspace_floats,
+                 (long long)(wp + wd));
+    float* parts = workspace;
+    float* direct = workspace + wp;
+    hipStream_t s = (hipStream_t)stream;
+    if (wd > 0) {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((wd + 255) / 256)), dim3(256), 0, s, direct, wd);
+        N3DS_LAUNCH_CHECK();
+    }
+    // one launch per class of scenes (same wavefronts per block); a class's blocks are consecutive
+    const int classes[4] = {8, 4, 2, 0};
+    for (int c = 0; c < 4; c++) {
+        int64_t b0 = -1, b1 = -1;
+        int sm = 0;
+        for (int i = 0; i < n_scenes; i++) {
+            const int32_t* e = scenes_host + (int64_t)i * SCENE_REC;
+            if (e[7] != classes[c] || e[5] == 0) continue;
+            if (b0 < 0 || e[4] < b0) b0 = e[4];
+            if (e[4] + e[5] > b1) b1 = e[4] + e[5];
+            if (e[3] > sm) sm = e[3];
+        }
+        if (b0 < 0) continue;
+        const BwdShape bs = bwd_shape(ns, color_id, sm, PG);
+        N3DS_REQUIRE(bs.lds_bytes <= g_max_lds, "backward needs %lld bytes of LDS per block",
+                     (long long)bs.lds_bytes);
+        RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
+                                 n_rays, n_samples, sm, shape_width, color_width, color_id, sdf_delta,
+                                 use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                                 adjoint_exact);
+        A.nch_stride = (s_max + 63) / 64;
+        A.scenes = scenes;
+        A.n_scenes = n_scenes;
+        A.block_base = (int)b0;
+        st = set_keep(A, const_cast<int*>(hit_dir), const_cast<int*>(hit_buf), const_cast<float*>(pair_buf),
+                      cap_hits, cap_pairs);
+        if (st) return st;
+        BwdKernel kern = bwd_kernel_ptr(ns, color_id);
+        if (bs.lds_bytes > 65536)
+            N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)bs.lds_bytes));
+        hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(64 * bs.waves), (size_t)bs.lds_bytes, s, A,
+                           grad_rgb, grad_acc, grad_depth, state, bs.use_lds ? parts : nullptr, bs.use_lds,
+                           bs.use_lds ? nullptr : direct);
+        N3DS_LAUNCH_CHECK();
+    }
+    const int64_t tot = (int64_t)n_strokes * PG;
+    hipLaunchKernelGGL(render_batch_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, parts,
+                       direct, scenes, n_scenes, n_strokes, shape_width, color_width, grad_shape, grad_color,
                        grad_density);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;

@@ -72,9 +72,11 @@ class _render_fn(Function):
     @staticmethod
     def forward(ctx, origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                 density_params, cfg):
-        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store) = cfg
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store, scenes) = cfg
         R, n1 = tdist.shape
         S, Pmax = shape_params.shape
+        if scenes is not None:
+            scenes.check(R, n1 - 1, S, Pmax, color_params.shape[1], color_id)
         for name, t in (('origins', origins), ('directions', directions), ('viewdirs', viewdirs), ('radii', radii),
                         ('tdist', tdist), ('shape_params', shape_params), ('color_params', color_params),
                         ('density_params', density_params)):
@@ -89,11 +91,16 @@ class _render_fn(Function):
         depth = torch.empty(R, dtype=torch.float32, device=dev)
         state = torch.empty(5, R * (n1 - 1), dtype=torch.float32, device=dev)
         need_grad = any(ctx.needs_input_grad[6:9])
-        keep = _keep_buffers(store, R, n1 - 1, S, color_id, dev) if need_grad else None
+        s_keep = S if scenes is None else scenes.s_max  # kept-hit layout follows the largest scene
+        keep = _keep_buffers(store, R, n1 - 1, s_keep, color_id, dev) if need_grad else None
         hd, hb, pb, cap_h, cap_p = keep if keep is not None else (None, None, None, 0, 0)
-        _lib.check(_lib.lib().n3ds_render_forward(
+        if scenes is None:
+            fwd, batch_args = _lib.lib().n3ds_render_forward, ()
+        else:
+            fwd, batch_args = _lib.lib().n3ds_render_batch_forward, scenes.c_args()
+        _lib.check(fwd(
             _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
-            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), R, n1 - 1, S, Pmax,
+            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax,
             color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
             _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), float(cull_eps), _lib.ptr(rgb), _lib.ptr(acc),
             _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.ptr(hd), _lib.ptr(hb), _lib.ptr(pb), cap_h,
@@ -111,7 +118,8 @@ class _render_fn(Function):
     def backward(ctx, g_rgb, g_acc, g_depth):
         (origins, directions, viewdirs, radii, tdist, table, shape_params, color_params, density_params,
          state, hd, hb, pb) = ctx.saved_tensors
-        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store) = ctx.cfg
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store,
+         scenes) = ctx.cfg
         cap_h, cap_p = ctx.caps
         R, n1 = tdist.shape
         S, Pmax = shape_params.shape
@@ -120,16 +128,23 @@ class _render_fn(Function):
         g_rgb = torch.zeros(R, 3, device=dev) if g_rgb is None else g_rgb.float().contiguous()
         g_acc = None if g_acc is None else g_acc.float().contiguous()
         g_depth = None if g_depth is None else g_depth.float().contiguous()
-        nws = _lib.lib().n3ds_render_backward_workspace_floats(R, S, Pmax, Q)
+        if scenes is None:
+            nws = _lib.lib().n3ds_render_backward_workspace_floats(R, S, Pmax, Q)
+            bwd, batch_args = _lib.lib().n3ds_render_backward, ()
+        else:
+            nws = _lib.lib().n3ds_render_batch_backward_workspace_floats(scenes.host_ptr(), scenes.n_scenes, n1 - 1,
+                                                                        Pmax, Q, color_id)
+            bwd, batch_args = _lib.lib().n3ds_render_batch_backward, scenes.c_args()
         if nws < 0:
             _lib.check(2)
         ws = torch.empty(max(int(nws), 1), dtype=torch.float32, device=dev)
         gs = torch.zeros_like(shape_params)
         gc = torch.zeros_like(color_params)
         gd = torch.zeros_like(density_params)
-        _lib.check(_lib.lib().n3ds_render_backward(
+        _lib.check(bwd(
             _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
-            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), R, n1 - 1, S, Pmax, Q, color_id,
+            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax, Q,
+            color_id,
             float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
             _lib.ctypes.cast(ctx.bgv, _lib.ctypes.c_void_p), float(cull_eps), 1 if adjoint == 'exact' else 0,
             _lib.ptr(g_rgb), _lib.ptr(g_acc), _lib.ptr(g_depth), _lib.ptr(state), _lib.ptr(hd), _lib.ptr(hb),
@@ -142,7 +157,7 @@ class _render_fn(Function):
 def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                    density_params, color_id=0, sdf_delta=1.0, use_laplace_transform=False, inv_scale_radius=False,
                    cone=False, coord_div=1.0, bg_color=(1.0, 1.0, 1.0), cull_eps=1e-7, adjoint=None,
-                   counters=None, store='auto'):
+                   counters=None, store='auto', scenes=None):
     """Render rays through a stroke field; returns (rgb [R,3], acc [R], depth [R]).
 
     origins/directions/viewdirs [R,3], radii [R] (sample radius = radii·t if ``cone``
@@ -158,6 +173,10 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
     (``keep_capacities``); ``'auto'``: ``True`` when that fits
     ``STORE_BUDGET_BYTES``; a ``(cap_hits, cap_pairs)`` tuple sets the per-ray
     capacities.  Results are identical in every mode.
+
+    ``scenes``: a ``multiscene.SceneBatch`` when the arrays hold many
+    independent scenes packed one after another (each ray composites only its
+    own scene's strokes); every stage then runs as one launch for all scenes.
     """
     if color_params.shape[1] != COLOR_WIDTH[color_id]:
         raise RuntimeError(f'color_params has width {color_params.shape[1]}, color_id {color_id} needs '
@@ -170,7 +189,7 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
                 raise RuntimeError(f'invalid stroke id {i}')
         stroke_ids = stroke_ids.to(device=shape_params.device, dtype=torch.int32)
     cfg = (int(color_id), float(sdf_delta), bool(use_laplace_transform), bool(inv_scale_radius), bool(cone),
-           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters, store)
+           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters, store, scenes)
     f = lambda t: t.contiguous().float()
     return _render_fn.apply(f(origins), f(directions), f(viewdirs), f(radii), f(tdist), stroke_ids.contiguous(),
                             f(shape_params), f(color_params), f(density_params), cfg)

@@ -189,3 +189,27 @@ def make_workload(name, seed=0, n_rays=None, n_samples=None, n_strokes=None):
     scene.update(rays)
     scene['types'] = types
     return scene
+
+
+def make_multiscene(n_scenes=32, seed=0, stroke_range=(20, 1000), ray_range=(1024, 65536), n_samples=128,
+                    shape_width=14):
+    """Independent scenes with log-uniform stroke and ray counts, each drawn like the
+    500-stroke mixed palette (40% cubic, 20% quadratic Bezier, 40% solids) with its own
+    cameras; shape params zero-padded to ``shape_width`` so the scenes pack together.
+
+    Returns (list of per-scene dicts, ray counts, stroke counts).
+    """
+    gen = torch.Generator().manual_seed(seed)
+    s_counts = [int(round(v)) for v in _logu(gen, n_scenes, lo=stroke_range[0], hi=stroke_range[1]).tolist()]
+    r_counts = [int(round(v)) for v in _logu(gen, n_scenes, lo=ray_range[0], hi=ray_range[1]).tolist()]
+    scenes = []
+    for S, R in zip(s_counts, r_counts):
+        types = palette(S, gen, 'cfg1')
+        sc = make_strokes(types, gen)
+        sp = sc['shape_params']
+        if sp.shape[1] < shape_width:
+            sc['shape_params'] = torch.cat([sp, sp.new_zeros(S, shape_width - sp.shape[1])], 1)
+        sc.update(make_rays(R, n_samples, gen))
+        sc['types'] = types
+        scenes.append(sc)
+    return scenes, r_counts, s_counts

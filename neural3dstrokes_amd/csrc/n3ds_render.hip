@@ -424,13 +424,16 @@ struct WaveMem {
     int* ci0;    // [HMAX] first sample of its interval
     int* coff;   // [HMAX] offset of its first pair among the pending pairs
     int* ccnt;   // [HMAX] its number of pairs
+    int* ccls;   // [HMAX] its stroke class (SDF family, see stroke_class)
+    int* xo;     // [64] class lists: first pair of the r-th hit of one class ...
+    int* xh;     // [64] ... and that hit
     float* pb;   // [npf][PBUF] per-pair values of the current window
     float* scr;  // [64][ng+1] per-lane gradient slots (adjoint)
     int* ul;     // [PBUF] window pairs with a nonzero adjoint (adjoint)
 };
 
 __host__ __device__ constexpr int wave_mem_floats(int ns, int npf, int ng) {
-    return 64 * ns + 4 * HMAX + npf * PBUF + (ng > 0 ? 64 * (ng + 1) + PBUF : 0);
+    return 64 * ns + 5 * HMAX + 128 + npf * PBUF + (ng > 0 ? 64 * (ng + 1) + PBUF : 0);
 }
 
 __device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf, int ng = 0) {
@@ -441,7 +444,10 @@ __device__ __forceinline__ WaveMem wave_mem(float* base, int ns, int npf, int ng
     w.ci0 = ib + HMAX;
     w.coff = ib + 2 * HMAX;
     w.ccnt = ib + 3 * HMAX;
-    w.pb = base + 64 * ns + 4 * HMAX;
+    w.ccls = ib + 4 * HMAX;
+    w.xo = ib + 5 * HMAX;
+    w.xh = ib + 5 * HMAX + 64;
+    w.pb = base + 64 * ns + 5 * HMAX + 128;
     w.scr = w.pb + npf * PBUF;
     w.ul = (int*)(w.scr + (ng > 0 ? 64 * (ng + 1) : 0));
     return w;
@@ -470,6 +476,42 @@ __device__ __forceinline__ int find_hit(const WaveMem& w, int c0, int c1, int wb
         if (m < c1 && w.coff[m] - wbase <= p) c = m;
     }
     return c;
+}
+
+// Pairs are evaluated class by class so that a 64-pair group runs one SDF
+// family: 0 cubic Bezier / Catmull-Rom, 1 quadratic Bezier, 2 solids.
+constexpr int NCLASS = 3;
+
+__device__ __forceinline__ int stroke_class(const float* rec, bool curve) {
+    return curve ? (ibits(rec[R_BASE]) == QUAD_BEZIER ? 1 : 0) : 2;
+}
+
+// Lists the hits of class x among the window's hits (lane h holds hit hc = c0 + h, of
+// class hcls; -1 past the window): w.xh[r] = r-th such hit, w.xo[r] = its first pair
+// in the class's pair sequence.  Returns the class's number of pairs; nx = hits.
+__device__ __forceinline__ int class_hits(const WaveMem& w, int hc, int hcls, int x, int lane, int& nx) {
+    const bool in = hcls == x;
+    const unsigned long long m = __ballot(in);
+    nx = __popcll(m);
+    if (!m) return 0;
+    const int cnt = in ? w.ccnt[hc] : 0;
+    const int incl = wave_incl_scan(cnt, lane);
+    if (in) {
+        const int r = __popcll(m & ((1ull << lane) - 1ull));
+        w.xo[r] = incl - cnt;
+        w.xh[r] = hc;
+    }
+    wave_sync();
+    return __shfl(incl, 63, 64);
+}
+
+// r-th hit of the current class list holding class pair pp
+__device__ __forceinline__ int class_find(const WaveMem& w, int nx, int pp) {
+    int r = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+        if (r + step < nx && w.xo[r + step] <= pp) r += step;
+    return r;
 }
 
 // Writes the ray's sample depths to LDS and returns whether they are sorted.
@@ -568,6 +610,7 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
         w.ci0[rank] = i0;
         w.coff[rank] = np + incl - cnt;
         w.ccnt[rank] = cnt;
+        w.ccls[rank] = stroke_class(A.table + (int64_t)k * REC, curve);
     }
     tot = __shfl(incl, 63, 64);
     wave_sync();
@@ -633,7 +676,7 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, 
             }
             if (keep && lane < nc) {
                 int* h = A.hbuf + (rh + hoff + lane) * 4;
-                h[0] = w.ck[nh + lane];
+                h[0] = w.ck[nh + lane] | (w.ccls[nh + lane] << 28);  // stroke | class << 28
                 h[1] = w.ci0[nh + lane];
                 h[2] = w.ccnt[nh + lane];
                 h[3] = w.coff[nh + lane] - np;  // offset within the chunk
@@ -652,13 +695,21 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, 
             const int wbase = w.coff[c0];
             const int c1 = window_end(w, c0, nh, wbase, lane);
             const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
-            // alpha of every (sample, stroke) pair of the window, one pair per lane
-            for (int p0 = 0; p0 < wp; p0 += 64) {
-                const int p = p0 + lane;
-                if (p < wp) {
-                    const int c = find_hit(w, c0, c1, wbase, p);
+            // alpha of every (sample, stroke) pair of the window, one pair per lane, class by class
+            const int hc = c0 + lane;
+            const int hcls = hc < c1 ? w.ccls[hc] : -1;
+            for (int xc = 0; xc < NCLASS; xc++) {
+              int nx;
+              const int xt = class_hits(w, hc, hcls, xc, lane, nx);
+              for (int p0 = 0; p0 < xt; p0 += 64) {
+                const int pp = p0 + lane;
+                if (pp < xt) {
+                    const int xr = class_find(w, nx, pp);
+                    const int c = w.xh[xr];
+                    const int j = pp - w.xo[xr];
+                    const int p = w.coff[c] - wbase + j;  // position in the window
                     const int k = w.ck[c];
-                    const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
+                    const int i = w.ci0[c] + j;
                     V3 x;
                     float r;
                     sample_point(A, rc, w.ts[i], x, r);
@@ -680,6 +731,8 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, 
                         }
                     }
                 }
+              }
+              if (xt) wave_sync();
             }
             wave_sync();
             // ordered over-compositing, strokes front to back
@@ -981,7 +1034,8 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                     int end = 0;
                     if (lane < nc) {
                         const int* h = A.hbuf + (rh + hoff + lane) * 4;
-                        w.ck[nh + lane] = h[0];
+                        w.ck[nh + lane] = h[0] & 0x0fffffff;
+                        w.ccls[nh + lane] = h[0] >> 28;
                         w.ci0[nh + lane] = h[1];
                         w.ccnt[nh + lane] = h[2];
                         w.coff[nh + lane] = np + h[3];
@@ -1015,12 +1069,20 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                         }
                     }
                 }
-                for (int p0 = 0; !gkept && p0 < wp; p0 += 64) {
-                    const int p = p0 + lane;
-                    if (p < wp) {
-                        const int c = find_hit(w, c0, c1, wbase, p);
+                const int hc = c0 + lane;
+                const int hcls = hc < c1 ? w.ccls[hc] : -1;
+                for (int xc = 0; !gkept && xc < NCLASS; xc++) {
+                  int nx;
+                  const int xt = class_hits(w, hc, hcls, xc, lane, nx);
+                  for (int p0 = 0; p0 < xt; p0 += 64) {
+                    const int pp = p0 + lane;
+                    if (pp < xt) {
+                        const int xr = class_find(w, nx, pp);
+                        const int c = w.xh[xr];
+                        const int j = pp - w.xo[xr];
+                        const int p = w.coff[c] - wbase + j;
                         const int k = w.ck[c];
-                        const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
+                        const int i = w.ci0[c] + j;
                         V3 x;
                         float r;
                         sample_point(A, rc, w.ts[i], x, r);
@@ -1034,6 +1096,8 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                             w.pb[3 * PBUF + p] = col.z;
                         }
                     }
+                  }
+                  if (xt) wave_sync();
                 }
                 wave_sync();
                 // (2) compositing adjoint, strokes front to back; per pair it leaves
@@ -1091,26 +1155,54 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 // (3) parameter gradients per pair, reduced per stroke.  Only pairs with a
                 //     nonzero adjoint contribute (alpha = 0 pairs do not): compact them first,
                 //     in order, so that each stroke's pairs stay contiguous
-                int nu = 0;
+                //     class by class (w.ul: class segments of (hit << 16 | pair), each in pair order)
+                int* code = (int*)w.scr;  // [wp] hit << 2 | class of a live pair, -1 otherwise
+                int xn[NCLASS] = {0, 0, 0};
                 for (int p0 = 0; p0 < wp; p0 += 64) {
                     const int p = p0 + lane;
                     const bool live = p < wp && (w.pb[p] != 0.f || w.pb[PBUF + p] != 0.f ||
                                                  w.pb[2 * PBUF + p] != 0.f || w.pb[3 * PBUF + p] != 0.f ||
                                                  w.pb[4 * PBUF + p] != 0.f);
-                    const unsigned long long lm = __ballot(live);
-                    if (live) w.ul[nu + __popcll(lm & ((1ull << lane) - 1ull))] = p;
-                    nu += __popcll(lm);
+                    int cd = -1;
+                    if (live) {
+                        const int c = find_hit(w, c0, c1, wbase, p);
+                        cd = (c << 2) | w.ccls[c];
+                    }
+                    if (p < wp) code[p] = cd;
+#pragma unroll
+                    for (int xc = 0; xc < NCLASS; xc++) xn[xc] += __popcll(__ballot(cd >= 0 && (cd & 3) == xc));
                 }
                 wave_sync();
-                for (int u0 = 0; u0 < nu; u0 += 64) {
+                int xs[NCLASS + 1];
+                xs[0] = 0;
+#pragma unroll
+                for (int xc = 0; xc < NCLASS; xc++) xs[xc + 1] = xs[xc] + xn[xc];
+                {
+                    int xf[NCLASS] = {xs[0], xs[1], xs[2]};
+                    for (int p0 = 0; p0 < wp; p0 += 64) {
+                        const int p = p0 + lane;
+                        const int cd = p < wp ? code[p] : -1;
+#pragma unroll
+                        for (int xc = 0; xc < NCLASS; xc++) {
+                            const bool in = cd >= 0 && (cd & 3) == xc;
+                            const unsigned long long m = __ballot(in);
+                            if (in) w.ul[xf[xc] + __popcll(m & ((1ull << lane) - 1ull))] = ((cd >> 2) << 16) | p;
+                            xf[xc] += __popcll(m);
+                        }
+                    }
+                }
+                wave_sync();
+                for (int xc = 0; xc < NCLASS; xc++)
+                for (int u0 = xs[xc]; u0 < xs[xc + 1]; u0 += 64) {
                     const int u = u0 + lane;
+                    const int nu = xs[xc + 1];
                     float G[NG];
 #pragma unroll
                     for (int g = 0; g < NG; g++) G[g] = 0.f;
                     int c = 0x7fffffff;  // sorts after every hit index
                     if (u < nu) {
-                        const int p = w.ul[u];
-                        c = find_hit(w, c0, c1, wbase, p);
+                        const int p = w.ul[u] & 0xffff;
+                        c = w.ul[u] >> 16;
                         const float ga = w.pb[p];
                         const float gdw = w.pb[PBUF + p];
                         const V3 gcol = v3(w.pb[2 * PBUF + p], w.pb[3 * PBUF + p], w.pb[4 * PBUF + p]);
@@ -1168,29 +1260,25 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                             }
                         }
                     }
-                    // segmented sum over the lanes of each stroke through an LDS transpose
-                    // (column NG of a lane's row holds its hit index; sorted over the lanes)
+                    // segmented sum over the lanes of each hit through an LDS transpose (lanes
+                    // hold nondecreasing hit indices; w.xo lists the first lane of each hit)
+                    const int nl = min(64, xs[xc + 1] - u0);
 #pragma unroll
                     for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
                     w.scr[lane * (NG + 1) + NG] = __int_as_float(c);
+                    const int cprev = __shfl_up(c, 1, 64);
+                    const bool head = lane < nl && (lane == 0 || c != cprev);
+                    const unsigned long long hm = __ballot(head);
+                    const int nseg = __popcll(hm);
+                    if (head) w.xo[__popcll(hm & ((1ull << lane) - 1ull))] = lane;
                     wave_sync();
-                    const int nl = min(64, nu - u0);
-                    const int cfirst = __builtin_amdgcn_readfirstlane(c);
-                    const int clast = __builtin_amdgcn_readlane(c, nl - 1);
-                    const int items = (clast - cfirst + 1) * NG;
+                    const int items = nseg * NG;
                     for (int it = lane; it < items; it += 64) {
                         const int jseg = it / NG;
                         const int g = it - jseg * NG;
-                        const int cc = cfirst + jseg;
-                        // lanes [lo, hi) hold the pairs of hit cc
-                        int lo = 0, hi = 0;
-#pragma unroll
-                        for (int step = 64; step > 0; step >>= 1) {
-                            if (lo + step <= nl && __float_as_int(w.scr[(lo + step - 1) * (NG + 1) + NG]) < cc)
-                                lo += step;
-                            if (hi + step <= nl && __float_as_int(w.scr[(hi + step - 1) * (NG + 1) + NG]) <= cc)
-                                hi += step;
-                        }
+                        const int lo = w.xo[jseg];
+                        const int hi = jseg + 1 < nseg ? w.xo[jseg + 1] : nl;
+                        const int cc = __float_as_int(w.scr[lo * (NG + 1) + NG]);
                         float sum = 0.f;
                         for (int l = lo; l < hi; l++) sum += w.scr[l * (NG + 1) + g];
                         if (sum != 0.f) {
@@ -1331,7 +1419,7 @@ static int device_info() {
 using namespace n3ds;
 
 static int render_check(int64_t R, int n, int S, int Pmax, int Q, int cid) {
-    N3DS_REQUIRE(R >= 0 && S >= 0, "negative size");
+    N3DS_REQUIRE(R >= 0 && S >= 0 && S < (1 << 28), "bad ray / stroke count");
     N3DS_REQUIRE(n >= 2 && n <= 256, "samples per ray must be in [2, 256] (got %d)", n);
     N3DS_REQUIRE(Pmax >= 1 && Pmax <= 23, "shape_params width %d out of range", Pmax);
     N3DS_REQUIRE(cid >= 0 && cid < 4, "color_id %d out of range", cid);

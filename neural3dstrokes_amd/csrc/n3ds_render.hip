@@ -57,11 +57,13 @@ enum : int {
     R_E1 = 32,      // 3: margin multiplier
     R_M = 36,       // 9: culling map diag(1/s) R^T (row-major)
     R_CP = 8,       // 14: curve params (curves reuse the solid slots)
-    R_BOX = 22,     // NPIECE * 6: piece boxes
+    R_PTS = 22,     // (NPIECE + 1) * 3: curve points at the piece ends
+    R_DEV = 73,     // NPIECE: bound on the distance of a piece from its chord
     R_HULL = 118,   // 6: whole-curve box
     R_RMAX = 124,   // max tube radius
 };
-static_assert(R_BOX + 6 * NPIECE <= R_HULL && R_RMAX < REC, "stroke record layout");
+static_assert(R_PTS + 3 * (NPIECE + 1) <= R_DEV && R_DEV + NPIECE <= R_HULL && R_RMAX < REC,
+              "stroke record layout");
 
 struct RenderArgs {
     const float* origins;
@@ -128,34 +130,34 @@ __device__ __forceinline__ float fbits(int i) { return __int_as_float(i); }
 // ---------------------------------------------------------------------------
 // stroke table
 // ---------------------------------------------------------------------------
-__device__ void curve_piece_box(int base, const V3 K[4], float a, float b, float* box) {
-    // Bezier control points of the piece [a, b] (Hermite form), whose hull bounds it
-    V3 c0, c1, c2, c3;
+// Bezier control points of the curve piece [a, b]; the piece lies in their convex hull
+__device__ void curve_piece_ctrl(int base, const V3 K[4], float a, float b, V3 c[4]) {
     if (base == QUAD_BEZIER) {
         // quadratic P(t) = K0 (1-t)^2 + 2 K1 t(1-t) + K2 t^2
         auto P = [&](float t) { float u = 1.f - t; return (u * u) * K[0] + (2.f * t * u) * K[1] + (t * t) * K[2]; };
         auto dP = [&](float t) { return (2.f * (1.f - t)) * (K[1] - K[0]) + (2.f * t) * (K[2] - K[1]); };
-        c0 = P(a);
-        c3 = P(b);
-        c1 = c0 + ((b - a) * 0.5f) * dP(a);
-        c2 = c1;
+        c[0] = P(a);
+        c[3] = P(b);
+        c[1] = c[0] + ((b - a) * 0.5f) * dP(a);
+        c[2] = c[1];
     } else {
         bool power = base == CATMULL_ROM;
         V3 pa = power ? curve_point<true>(a, K) : curve_point<false>(a, K);
         V3 pb = power ? curve_point<true>(b, K) : curve_point<false>(b, K);
         V3 ta = power ? curve_tangent<true>(a, K) : curve_tangent<false>(a, K);
         V3 tb = power ? curve_tangent<true>(b, K) : curve_tangent<false>(b, K);
-        c0 = pa;
-        c3 = pb;
-        c1 = pa + ((b - a) / 3.f) * ta;
-        c2 = pb - ((b - a) / 3.f) * tb;
+        c[0] = pa;
+        c[3] = pb;
+        c[1] = pa + ((b - a) / 3.f) * ta;
+        c[2] = pb - ((b - a) / 3.f) * tb;
     }
-    box[0] = fminf(fminf(c0.x, c1.x), fminf(c2.x, c3.x));
-    box[1] = fminf(fminf(c0.y, c1.y), fminf(c2.y, c3.y));
-    box[2] = fminf(fminf(c0.z, c1.z), fminf(c2.z, c3.z));
-    box[3] = fmaxf(fmaxf(c0.x, c1.x), fmaxf(c2.x, c3.x));
-    box[4] = fmaxf(fmaxf(c0.y, c1.y), fmaxf(c2.y, c3.y));
-    box[5] = fmaxf(fmaxf(c0.z, c1.z), fmaxf(c2.z, c3.z));
+}
+
+__device__ float point_segment_dist(V3 p, V3 a, V3 b) {
+    const V3 ab = b - a, ap = p - a;
+    const float l2 = dot(ab, ab);
+    const float h = l2 > 0.f ? fminf(fmaxf(dot(ap, ab) / l2, 0.f), 1.f) : 0.f;
+    return safe_len(ap - ab * h);
 }
 
 __global__ void render_prep_kernel(float* __restrict__ table, const int* __restrict__ ids,
@@ -227,17 +229,33 @@ __global__ void render_prep_kernel(float* __restrict__ table, const int* __restr
             for (int j = 0; j < 4; j++) K[j] = c.K[j];
             r1 = row[12]; r2 = row[13];
         }
+        // pieces as chord capsules: the piece [j, j+1] / NPIECE lies within dev_j of the chord
+        // between its end points (the distance to a segment is convex, so its maximum over the
+        // hull of the piece's control points is at a control point); the hull of all control
+        // points bounds the whole curve
         float hull[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
         for (int j = 0; j < NPIECE; j++) {
-            float* box = rec + R_BOX + 6 * j;
-            curve_piece_box(base, K, (float)j / NPIECE, (float)(j + 1) / NPIECE, box);
-            float pad = 1e-5f * (1.f + fmaxf(fmaxf(fabsf(box[0]), fabsf(box[3])), fmaxf(fabsf(box[2]), fabsf(box[5]))));
-            for (int a = 0; a < 3; a++) {
-                box[a] -= pad;
-                box[3 + a] += pad;
-                hull[a] = fminf(hull[a], box[a]);
-                hull[3 + a] = fmaxf(hull[3 + a], box[3 + a]);
+            V3 c[4];
+            curve_piece_ctrl(base, K, (float)j / NPIECE, (float)(j + 1) / NPIECE, c);
+            float mag = 0.f;
+            for (int q = 0; q < 4; q++) {
+                hull[0] = fminf(hull[0], c[q].x); hull[1] = fminf(hull[1], c[q].y); hull[2] = fminf(hull[2], c[q].z);
+                hull[3] = fmaxf(hull[3], c[q].x); hull[4] = fmaxf(hull[4], c[q].y); hull[5] = fmaxf(hull[5], c[q].z);
+                mag = fmaxf(mag, fmaxf(fabsf(c[q].x), fmaxf(fabsf(c[q].y), fabsf(c[q].z))));
             }
+            const float dev = fmaxf(point_segment_dist(c[1], c[0], c[3]), point_segment_dist(c[2], c[0], c[3]));
+            rec[R_DEV + j] = dev * 1.001f + 1e-5f * (1.f + mag);
+            rec[R_PTS + 3 * j + 0] = c[0].x; rec[R_PTS + 3 * j + 1] = c[0].y; rec[R_PTS + 3 * j + 2] = c[0].z;
+            if (j == NPIECE - 1) {
+                rec[R_PTS + 3 * NPIECE + 0] = c[3].x;
+                rec[R_PTS + 3 * NPIECE + 1] = c[3].y;
+                rec[R_PTS + 3 * NPIECE + 2] = c[3].z;
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            const float pad = 1e-5f * (1.f + fmaxf(fabsf(hull[a]), fabsf(hull[3 + a])));
+            hull[a] -= pad;
+            hull[3 + a] += pad;
         }
         for (int a = 0; a < 6; a++) rec[R_HULL + a] = hull[a];
         rec[R_RMAX] = fmaxf(fmaxf(r1, r2), 0.f) * 1.0001f + 1e-6f;
@@ -265,6 +283,76 @@ __device__ __forceinline__ void pad_interval(float t0, float t1, float& ta, floa
     float pad = 1e-4f * (t1 - t0) + 1e-5f * (fabsf(t0) + fabsf(t1)) + 1e-6f;
     ta = t0 - pad;
     tb = t1 + pad;
+}
+
+// Order-preserving map of floats to ints (for LDS integer min / max) and back.
+__device__ __forceinline__ int ord_int(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float ord_float(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+// Ray interval [t0, t1] within distance e of the segment [a, b] (a capsule, so the
+// interval is convex): hull of the intervals in the two end spheres and in the
+// cylinder part.  idd = 1 / |d|^2.  Hardware reciprocals and square roots (1 ulp);
+// e is padded relatively so rounding cannot drop a boundary point.
+__device__ __forceinline__ bool capsule_interval(V3 o, V3 d, float idd, V3 a, V3 b, float e, float& t0, float& t1) {
+    e = e * 1.0002f + 1e-6f;
+    const float e2 = e * e;
+    const V3 ba = b - a, oa = o - a, ob1 = o - b;
+    const float l2 = dot(ba, ba), dd = dot(d, d);
+    const float db = dot(d, ba), ob = dot(oa, ba);
+    float lo = 1e30f, hi = -1e30f;
+    // end spheres
+#pragma unroll
+    for (int end = 0; end < 2; end++) {
+        const V3 q0 = end ? ob1 : oa;
+        const float tc = -dot(q0, d) * idd;
+        const V3 q = q0 + d * tc;
+        const float h = e2 - dot(q, q);
+        if (h >= 0.f) {
+            const float w = __builtin_amdgcn_sqrtf(h * idd);
+            lo = fminf(lo, tc - w);
+            hi = fmaxf(hi, tc + w);
+        }
+    }
+    // cylinder around the chord, between the planes through a and b
+    if (l2 > 0.f) {
+        const float il2 = __builtin_amdgcn_rcpf(l2);
+        const V3 dp = d - ba * (db * il2), op = oa - ba * (ob * il2);  // components normal to the axis
+        const float A2 = dot(dp, dp), B2 = dot(op, dp), C2 = dot(op, op) - e2;
+        float c0 = -1e30f, c1 = 1e30f;
+        bool any = true;
+        if (A2 > 1e-12f * dd) {
+            const float disc = B2 * B2 - A2 * C2;
+            if (disc >= 0.f) {
+                const float sq = __builtin_amdgcn_sqrtf(disc), ia = __builtin_amdgcn_rcpf(A2);
+                c0 = (-B2 - sq) * ia;
+                c1 = (-B2 + sq) * ia;
+            } else {
+                any = false;
+            }
+        } else {
+            any = C2 <= 0.f;
+        }
+        if (any) {
+            if (fabsf(db) > 1e-12f * sqrtf(dd * l2)) {
+                const float idb = __builtin_amdgcn_rcpf(db);
+                const float s0 = -ob * idb, s1 = (l2 - ob) * idb;
+                c0 = fmaxf(c0, fminf(s0, s1));
+                c1 = fminf(c1, fmaxf(s0, s1));
+            } else if (ob < 0.f || ob > l2) {
+                any = false;
+            }
+            if (any && c0 <= c1) {
+                lo = fminf(lo, c0);
+                hi = fmaxf(hi, c1);
+            }
+        }
+    }
+    t0 = lo;
+    t1 = hi;
+    return lo <= hi;
 }
 
 // Ray interval [ta, tb] inside the stroke's bound.  Solids: final.  Curves
@@ -363,6 +451,7 @@ struct RayConst {
     V3 o, d, vd, oc, dc;  // oc, dc: origin / direction in the (warped) stroke coordinates
     V3 di;                // 1 / dc componentwise
     float radii, tlo, thi, m;
+    float idd;            // 1 / |dc|^2
 };
 
 __device__ __forceinline__ RayConst load_ray(const RenderArgs& A, int64_t ray) {
@@ -383,6 +472,7 @@ __device__ __forceinline__ RayConst load_ray(const RenderArgs& A, int64_t ray) {
         c.dc = v3(c.d.x / A.coord_div, c.d.y / A.coord_div, c.d.z / A.coord_div);
     }
     c.di = v3(1.f / c.dc.x, 1.f / c.dc.y, 1.f / c.dc.z);
+    c.idd = 1.f / dot(c.dc, c.dc);
     return c;
 }
 
@@ -548,32 +638,40 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
     bool curve = false;
     bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi,
                                      ta, tb, curve);
-    // curves whose whole-curve box is hit: test their NPIECE piece boxes with one
-    // lane per piece (64 / NPIECE curves at a time), the union of the hit pieces
-    // is the interval (pb is free outside the evaluation windows)
+    // curves whose whole-curve box is hit: their NPIECE pieces are tested against the
+    // piece's chord box with one lane per piece (64 / NPIECE curves at a time).  The
+    // hit carries the union of the passing boxes as a provisional interval and their
+    // mask; refine_pending narrows it to the chord capsules when the pending hits are
+    // evaluated (many chunks at once).  w.pb is free outside the evaluation windows.
     static_assert(NPIECE == 16, "one 16-lane group per curve");
     const unsigned long long cm = __ballot(hit && curve);
+    int pmask = 0;
     if (cm) {
-        int* src = (int*)w.pb;      // [64] lane of the c-th curve to refine
-        float* res = w.pb + 64;     // [64][2] refined interval
+        int* src = (int*)w.pb;      // [64] lane of the c-th curve
+        float* res = w.pb + 64;     // [64][2] box-union interval
+        int* msk = (int*)w.pb + 192;  // [64] passing pieces
         const int crank = __popcll(cm & ((1ull << lane) - 1ull));
         if (hit && curve) src[crank] = lane;
         wave_sync();
         const int ncv = __popcll(cm);
+        const int kb = A.S - 1 - base0;  // stroke of lane 0 of this chunk
         for (int q0 = 0; q0 < ncv; q0 += 64 / NPIECE) {
             const int cs = q0 + lane / NPIECE;
             const int piece = lane % NPIECE;
             float u0 = 1e30f, u1 = -1e30f;
+            bool bh = false;
             if (cs < ncv) {
-                const float* rec = A.table + (int64_t)(A.S - 1 - (base0 + src[cs])) * REC;
-                const float e = rec[R_RMAX] + rc.m;
-                const float* bx = rec + R_BOX + 6 * piece;
+                const float* rec = A.table + (int64_t)(kb - src[cs]) * REC;
+                const float e = rec[R_RMAX] + rc.m + rec[R_DEV + piece];
+                const V3 a = ld3(rec + R_PTS + 3 * piece), b = ld3(rec + R_PTS + 3 * piece + 3);
                 float p0 = rc.tlo, p1 = rc.thi;
-                slab_inv(rc.oc.x, rc.di.x, bx[0] - e, bx[3] + e, p0, p1);
-                slab_inv(rc.oc.y, rc.di.y, bx[1] - e, bx[4] + e, p0, p1);
-                slab_inv(rc.oc.z, rc.di.z, bx[2] - e, bx[5] + e, p0, p1);
-                if (p0 <= p1) { u0 = p0; u1 = p1; }
+                slab_inv(rc.oc.x, rc.di.x, fminf(a.x, b.x) - e, fmaxf(a.x, b.x) + e, p0, p1);
+                slab_inv(rc.oc.y, rc.di.y, fminf(a.y, b.y) - e, fmaxf(a.y, b.y) + e, p0, p1);
+                slab_inv(rc.oc.z, rc.di.z, fminf(a.z, b.z) - e, fmaxf(a.z, b.z) + e, p0, p1);
+                bh = p0 <= p1;
+                if (bh) { u0 = p0; u1 = p1; }
             }
+            const unsigned long long bm = __ballot(bh);
 #pragma unroll
             for (int off = NPIECE / 2; off > 0; off >>= 1) {
                 u0 = fminf(u0, __shfl_xor(u0, off, 64));
@@ -582,11 +680,13 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
             if (cs < ncv && piece == 0) {
                 res[2 * cs] = u0;
                 res[2 * cs + 1] = u1;
+                msk[cs] = (int)((bm >> (lane & ~(NPIECE - 1))) & 0xffffull);
             }
         }
         wave_sync();
         if (hit && curve) {
             const float u0 = res[2 * crank], u1 = res[2 * crank + 1];
+            pmask = msk[crank];
             hit = u0 <= u1;
             if (hit) pad_interval(u0, u1, ta, tb);
         }
@@ -610,7 +710,8 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
         w.ci0[rank] = i0;
         w.coff[rank] = np + incl - cnt;
         w.ccnt[rank] = cnt;
-        w.ccls[rank] = stroke_class(A.table + (int64_t)k * REC, curve);
+        // class | provisional (curve still to refine) << 2 | passing pieces << 16
+        w.ccls[rank] = stroke_class(A.table + (int64_t)k * REC, curve) | (curve ? 4 | (pmask << 16) : 0);
     }
     tot = __shfl(incl, 63, 64);
     wave_sync();
@@ -624,11 +725,85 @@ __device__ __forceinline__ int window_end(const WaveMem& w, int c0, int nh, int 
     return c0 + __popcll(__ballot(fits));
 }
 
+// Number of pending hits (in compositing order) whose stroke lies in a chunk
+// before chunk cc; top = index of the first stroke in compositing order.
+__device__ __forceinline__ int first_hit_of_chunk(const WaveMem& w, int nh, int top, int cc) {
+    int lo = 0;
+#pragma unroll
+    for (int step = HMAX; step > 0; step >>= 1)
+        if (lo + step <= nh && ((top - w.ck[lo + step - 1]) >> 6) < cc) lo += step;
+    return lo;
+}
+
+// Narrows the provisional curve hits among the nh pending hits to their chord
+// capsules (each hit lane tests its passing pieces one after another), recomputes
+// their samples, drops hits left without samples and recomputes the pair offsets.
+// Returns the new number of pending hits; np receives the pending pairs.
+template <int NS>
+__device__ __forceinline__ int refine_pending(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
+                                              int nh, int lane, int& np) {
+    int out = 0, pairs = 0;
+    for (int h0 = 0; h0 < nh; h0 += 64) {
+        const int c = h0 + lane;
+        int k = 0, i0 = 0, cnt = 0, cls = 0;
+        if (c < nh) {
+            k = w.ck[c];
+            i0 = w.ci0[c];
+            cnt = w.ccnt[c];
+            cls = w.ccls[c];
+        }
+        if (c < nh && (cls & 4)) {
+            const float* rec = A.table + (int64_t)k * REC;
+            const float e0 = rec[R_RMAX] + rc.m;
+            float u0 = 1e30f, u1 = -1e30f;
+            for (unsigned m = (unsigned)cls >> 16; m; m &= m - 1) {
+                const int pc = __builtin_ctz(m);
+                float p0, p1;
+                if (capsule_interval(rc.oc, rc.dc, rc.idd, ld3(rec + R_PTS + 3 * pc), ld3(rec + R_PTS + 3 * pc + 3),
+                                     e0 + rec[R_DEV + pc], p0, p1)) {
+                    u0 = fminf(u0, fmaxf(p0, rc.tlo));
+                    u1 = fmaxf(u1, fminf(p1, rc.thi));
+                }
+            }
+            cnt = 0;
+            if (u0 <= u1) {
+                float ta, tb;
+                pad_interval(u0, u1, ta, tb);
+                if (sorted) {
+                    i0 = count_below<NS, false>(w.ts, ta);
+                    cnt = count_below<NS, true>(w.ts, tb) - i0;
+                } else {
+                    i0 = 0;
+                    cnt = A.n;
+                }
+            }
+            cls &= 3;
+        }
+        const bool keep = c < nh && cnt > 0;
+        const unsigned long long km = __ballot(keep);
+        const int incl = wave_incl_scan(keep ? cnt : 0, lane);
+        wave_sync();  // every lane has read its entry before any is overwritten (new index <= old)
+        if (keep) {
+            const int r = out + __popcll(km & ((1ull << lane) - 1ull));
+            w.ck[r] = k;
+            w.ci0[r] = i0;
+            w.ccnt[r] = cnt;
+            w.ccls[r] = cls;
+            w.coff[r] = pairs + incl - cnt;
+        }
+        out += __popcll(km);
+        pairs += __shfl(incl, 63, 64);
+        wave_sync();
+    }
+    np = pairs;
+    return out;
+}
+
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
 template <int NS, int CID>
-__global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, float* __restrict__ rgb_out,
+__global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 8))) void render_fwd_kernel(RenderArgs A0, float* __restrict__ rgb_out,
                                                                  float* __restrict__ acc_out,
                                                                  float* __restrict__ depth_out,
                                                                  float* __restrict__ state,
@@ -653,43 +828,52 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, 
 #pragma unroll
     for (int s = 0; s < NS; s++) { T[s] = 1.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
     unsigned long long my_pos = 0, my_ev = 0, my_vis = 0;  // lane b counts stroke type b (counting pass)
-    // cursors into this ray's region of the kept hits / pair values; once a chunk
-    // does not fit, it and every later chunk are left to the adjoint to recompute
+    // cursors into this ray's region of the kept hits / pair values; once a group of
+    // chunks does not fit, it and every later chunk are left to the adjoint to recompute
     int hoff = 0, poff = 0;
     bool full = false;
     const int64_t rh = ray * (int64_t)A.cap_h;
     const int64_t rp = ray * (int64_t)A.cap_p * NPF;
-    int nh = 0, np = 0;               // pending hits / pairs
-    int kept_np = 0, group_poff = 0;  // pending pairs that are kept (a prefix) and their offset in the region
+    int nh = 0, np = 0;  // pending hits / pairs
+    int gch = 0;         // first chunk of the pending group
     for (int ch = 0; ch < A.nch; ch++) {
-        if (nh == 0) group_poff = poff;
         int tot;
         const int nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
-        if (A.hdir) {
-            const bool keep = !full && hoff + nc <= A.cap_h && poff + tot <= A.cap_p;
-            full = !keep;
-            if (lane == 0) {
-                int* e = A.hdir + (ray * A.nch_stride + ch) * 3;
-                e[0] = keep ? hoff : -1;
-                e[1] = poff;
-                e[2] = nc;
-            }
-            if (keep && lane < nc) {
-                int* h = A.hbuf + (rh + hoff + lane) * 4;
-                h[0] = w.ck[nh + lane] | (w.ccls[nh + lane] << 28);  // stroke | class << 28
-                h[1] = w.ci0[nh + lane];
-                h[2] = w.ccnt[nh + lane];
-                h[3] = w.coff[nh + lane] - np;  // offset within the chunk
-            }
-            if (keep) {
-                hoff += nc;
-                poff += tot;
-                kept_np += tot;
-            }
-        }
         nh += nc;
         np += tot;
         if (nh <= HMAX - 64 && np < FLUSH_PAIRS && ch + 1 < A.nch) continue;
+        nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
+        int kept_np = 0;
+        const int group_poff = poff;
+        if (A.hdir) {
+            const bool keep = !full && hoff + nh <= A.cap_h && poff + np <= A.cap_p;
+            full = !keep;
+            // per chunk of the group: first hit, absolute offset of its first pair, hits
+            for (int cb = gch; cb <= ch; cb += 64) {
+                const int cc = cb + lane;
+                if (cc <= ch) {
+                    const int f = first_hit_of_chunk(w, nh, A.S - 1, cc);
+                    const int f1 = first_hit_of_chunk(w, nh, A.S - 1, cc + 1);
+                    int* e = A.hdir + (ray * A.nch_stride + cc) * 3;
+                    e[0] = keep ? hoff + f : -1;
+                    e[1] = poff + (f < nh ? w.coff[f] : np);
+                    e[2] = f1 - f;
+                }
+            }
+            if (keep) {
+                for (int c = lane; c < nh; c += 64) {
+                    int* h = A.hbuf + (rh + hoff + c) * 4;
+                    h[0] = w.ck[c] | ((w.ccls[c] & 3) << 28);  // stroke | class << 28
+                    h[1] = w.ci0[c];
+                    h[2] = w.ccnt[c];
+                    h[3] = poff + w.coff[c];  // absolute pair offset in the ray's region
+                }
+                hoff += nh;
+                poff += np;
+                kept_np = np;
+            }
+        }
+        gch = ch + 1;
         // evaluate and composite the pending hits, window by window
         for (int c0 = 0; c0 < nh;) {
             const int wbase = w.coff[c0];
@@ -697,7 +881,7 @@ __global__ __launch_bounds__(64 * FWAVES) void render_fwd_kernel(RenderArgs A0, 
             const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
             // alpha of every (sample, stroke) pair of the window, one pair per lane, class by class
             const int hc = c0 + lane;
-            const int hcls = hc < c1 ? w.ccls[hc] : -1;
+            const int hcls = hc < c1 ? (w.ccls[hc] & 3) : -1;
             for (int xc = 0; xc < NCLASS; xc++) {
               int nx;
               const int xt = class_hits(w, hc, hcls, xc, lane, nx);
@@ -1035,11 +1219,11 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                     if (lane < nc) {
                         const int* h = A.hbuf + (rh + hoff + lane) * 4;
                         w.ck[nh + lane] = h[0] & 0x0fffffff;
-                        w.ccls[nh + lane] = h[0] >> 28;
+                        w.ccls[nh + lane] = (h[0] >> 28) & 3;
                         w.ci0[nh + lane] = h[1];
                         w.ccnt[nh + lane] = h[2];
-                        w.coff[nh + lane] = np + h[3];
-                        end = h[3] + h[2];
+                        w.coff[nh + lane] = np + (h[3] - kpoff);
+                        end = h[3] - kpoff + h[2];
                     }
                     tot = nc > 0 ? __shfl(end, nc - 1, 64) : 0;
                     wave_sync();
@@ -1052,6 +1236,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 do_flush = nh > HMAX - 64 || np >= FLUSH_PAIRS || ch == A.nch;
             }
             if (!do_flush) continue;
+            if (!gkept) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
             for (int c0 = 0; c0 < nh;) {
                 const int wbase = w.coff[c0];
                 const int c1 = window_end(w, c0, nh, wbase, lane);
@@ -1070,7 +1255,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                     }
                 }
                 const int hc = c0 + lane;
-                const int hcls = hc < c1 ? w.ccls[hc] : -1;
+                const int hcls = hc < c1 ? (w.ccls[hc] & 3) : -1;
                 for (int xc = 0; !gkept && xc < NCLASS; xc++) {
                   int nx;
                   const int xt = class_hits(w, hc, hcls, xc, lane, nx);
@@ -1166,7 +1351,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                     int cd = -1;
                     if (live) {
                         const int c = find_hit(w, c0, c1, wbase, p);
-                        cd = (c << 2) | w.ccls[c];
+                        cd = (c << 2) | (w.ccls[c] & 3);
                     }
                     if (p < wp) code[p] = cd;
 #pragma unroll

@@ -245,7 +245,7 @@ def memory_block(steps=3, warmup=1):
     import torch
     from neural3dstrokes_amd import render, scenes
     dev = torch.device('cuda:0')
-    sc = scenes.make_workload('cfg4', seed=0)
+    sc = scenes.make_workload('palette1000_640000x192', seed=0)
     R, n1 = sc['tdist'].shape
     g = torch.Generator().manual_seed(1)
     gr, ga = torch.randn(R, 3, generator=g).to(dev), torch.randn(R, generator=g).to(dev)
@@ -373,7 +373,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='cfg1')
+    ap.add_argument('--workload', default='palette500_65536x128')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--adjoint', default='exact', choices=['exact', 'reference'])
     ap.add_argument('--plain-rays', type=int, default=8192, help='rays for the plain per-op comparator (0: skip)')

@@ -148,28 +148,28 @@ def make_rays(n_rays, n_samples, gen, n_cams=64, cam_radius=4.0, fov_deg=40.0, n
                 radii=torch.full((n_rays,), radius, dtype=torch.float32), tdist=tdist)
 
 
-SOLIDS_CFG0 = [0, 1, 4]          # ellipsoid, obb, scapsule
-SOLIDS_CFG1 = [0, 1, 2, 7, 8]    # sphere, cube, round-cube, octahedron, tetrahedron (per-axis scale)
+SOLIDS_ELLIPSOID_OBB_CAPSULE = [0, 1, 4]          # ellipsoid, obb, scapsule
+SOLIDS_MIXED = [0, 1, 2, 7, 8]    # sphere, cube, round-cube, octahedron, tetrahedron (per-axis scale)
 
 
 def palette(n_strokes, gen, kind):
     """Seeded type assignment (a permutation of the type multiset) in draw order."""
-    if kind == 'cfg0':
-        types = [SOLIDS_CFG0[int(i)] for i in torch.randint(0, 3, (n_strokes,), generator=gen)]
+    if kind == 'solids':
+        types = [SOLIDS_ELLIPSOID_OBB_CAPSULE[int(i)] for i in torch.randint(0, 3, (n_strokes,), generator=gen)]
         return types
     n_cubic = int(round(0.4 * n_strokes))
     n_quad = int(round(0.2 * n_strokes))
     n_solid = n_strokes - n_cubic - n_quad
-    solids = [SOLIDS_CFG1[int(i)] for i in torch.randint(0, 5, (n_solid,), generator=gen)]
+    solids = [SOLIDS_MIXED[int(i)] for i in torch.randint(0, 5, (n_solid,), generator=gen)]
     types = [10] * n_cubic + [9] * n_quad + solids
     perm = torch.randperm(n_strokes, generator=gen)
     return [types[int(i)] for i in perm]
 
 
 WORKLOADS = {
-    'cfg0': dict(n_strokes=50, n_rays=4096, n_samples=64, palette='cfg0'),
-    'cfg1': dict(n_strokes=500, n_rays=65536, n_samples=128, palette='cfg1'),
-    'cfg4': dict(n_strokes=1000, n_rays=640000, n_samples=192, palette='cfg1'),
+    'solids50_4096x64': dict(n_strokes=50, n_rays=4096, n_samples=64, palette='solids'),
+    'palette500_65536x128': dict(n_strokes=500, n_rays=65536, n_samples=128, palette='mixed'),
+    'palette1000_640000x192': dict(n_strokes=1000, n_rays=640000, n_samples=192, palette='mixed'),
 }
 
 
@@ -204,7 +204,7 @@ def make_multiscene(n_scenes=32, seed=0, stroke_range=(20, 1000), ray_range=(102
     r_counts = [int(round(v)) for v in _logu(gen, n_scenes, lo=ray_range[0], hi=ray_range[1]).tolist()]
     scenes = []
     for S, R in zip(s_counts, r_counts):
-        types = palette(S, gen, 'cfg1')
+        types = palette(S, gen, 'mixed')
         sc = make_strokes(types, gen)
         sp = sc['shape_params']
         if sp.shape[1] < shape_width:

@@ -41,12 +41,12 @@ def case(name, seed, n_strokes, radius):
 
 def main():
     data = {}
-    data.update(case('cfg0', 11, 12, 0.05))
-    data.update(case('cfg1', 12, 16, 0.05))
+    data.update(case('solids50_4096x64', 11, 12, 0.05))
+    data.update(case('palette500_65536x128', 12, 16, 0.05))
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'field_small.npz')
     np.savez_compressed(path, **data)
     print(f'wrote {path}: ' + ', '.join(f'{k} {v.shape}' for k, v in data.items() if k.endswith(('rgb', 'acc'))))
-    for c in ('cfg0', 'cfg1'):
+    for c in ('solids50_4096x64', 'palette500_65536x128'):
         print(c, 'acc > 0.01 on', int((data[f'{c}_acc'] > 0.01).sum()), 'rays')
 
 

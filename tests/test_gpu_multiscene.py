@@ -26,7 +26,7 @@ def _batch_case(n_scenes, seed, n_samples, strokes, rays, extra_empty=True):
         empty = dict(stroke_ids=torch.zeros(0, dtype=torch.int32), shape_params=torch.zeros(0, 14),
                      color_params=torch.zeros(0, 3), density_params=torch.zeros(0))
         empty.update(r)
-        s2 = W.make_strokes(W.palette(25, g, 'cfg1'), g)
+        s2 = W.make_strokes(W.palette(25, g, 'mixed'), g)
         s2['shape_params'] = torch.cat([s2['shape_params'], torch.zeros(25, 14 - s2['shape_params'].shape[1])], 1)
         s2.update({k: v[:0] for k, v in W.make_rays(1, n_samples, g).items()})
         sc = sc[:1] + [empty] + sc[1:2] + [s2] + sc[2:]

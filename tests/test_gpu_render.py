@@ -41,8 +41,8 @@ def _oracle_args(sc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('name,n_rays,n_samples,n_strokes', [('cfg0', 96, 64, None), ('cfg1', 48, 128, 120),
-                                                             ('cfg1', 24, 192, 60)])
+@pytest.mark.parametrize('name,n_rays,n_samples,n_strokes', [('solids50_4096x64', 96, 64, None), ('palette500_65536x128', 48, 128, 120),
+                                                             ('palette500_65536x128', 24, 192, 60)])
 def test_render_exact(name, n_rays, n_samples, n_strokes, dev):
     sc = _scene(name, n_rays, n_samples, n_strokes)
     (rgb, acc, depth), (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev)
@@ -62,7 +62,7 @@ def test_render_exact(name, n_rays, n_samples, n_strokes, dev):
 def test_render_kept_activations(store, color_id, dev):
     """Backward from kept hits / alpha, from recomputation, and mixed (tiny per-ray
     capacities: early chunks kept, later ones recomputed) give the same gradients."""
-    sc = _scene('cfg1', 48, 128, 150, seed=3)
+    sc = _scene('palette500_65536x128', 48, 128, 150, seed=3)
     if color_id == 1:
         g = torch.Generator().manual_seed(5)
         sc['color_params'] = torch.cat([torch.rand(150, 6, generator=g) - 0.5, torch.rand(150, 6, generator=g)], -1)
@@ -80,7 +80,7 @@ def test_render_kept_activations(store, color_id, dev):
 
 @pytest.mark.gpu
 def test_render_reference_adjoint(dev):
-    sc = _scene('cfg1', 32, 128, 80, seed=3)
+    sc = _scene('palette500_65536x128', 32, 128, 80, seed=3)
     _, (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev, adjoint='reference', seed=3)
     rs, rc, rd = ofield.render_field_grads_reference(g_rgb.to(D64), g_acc.to(D64), *_oracle_args(sc))
     assert rel_err(gs, rs) < 1e-4
@@ -90,7 +90,7 @@ def test_render_reference_adjoint(dev):
 
 @pytest.mark.gpu
 def test_render_laplace_cone_warp(dev):
-    sc = _scene('cfg0', 64, 64, seed=4)
+    sc = _scene('solids50_4096x64', 64, 64, seed=4)
     sc['radii'] = torch.full_like(sc['radii'], 0.004)
     kw = dict(use_laplace_transform=True, cone=True, coord_div=1.5, sdf_delta=2.0)
     (rgb, acc, depth), (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev, seed=4, **kw)
@@ -106,7 +106,7 @@ def test_render_laplace_cone_warp(dev):
 def test_render_matches_dropin_chain(dev):
     """Fused op == per-type drop-in stroke_fn + compose_strokes + torch ray march on the GPU."""
     from neural3dstrokes_amd.strokelib import strokes as dstrokes
-    sc = _scene('cfg1', 40, 128, 100, seed=5)
+    sc = _scene('palette500_65536x128', 40, 128, 100, seed=5)
     rgb, acc, _ = render.render_strokes(sc['origins'].to(dev), sc['directions'].to(dev), sc['viewdirs'].to(dev),
                                         sc['radii'].to(dev), sc['tdist'].to(dev), sc['stroke_ids'],
                                         sc['shape_params'].to(dev), sc['color_params'].to(dev),
@@ -139,7 +139,7 @@ def test_render_matches_dropin_chain(dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('case', ['cfg0', 'cfg1'])
+@pytest.mark.parametrize('case', ['solids50_4096x64', 'palette500_65536x128'])
 def test_render_golden_fixture(case, dev):
     """HIP renderer against the committed float64 fixture tests/golden/field_small.npz."""
     import os

@@ -90,7 +90,7 @@ def test_grid_encoder_bookkeeping_matches_reference_layout():
 
 def test_product_path_rejects_cpu_tensors():
     from neural3dstrokes_amd import render
-    sc = scenes.make_workload('cfg0', seed=0, n_rays=4, n_samples=8, n_strokes=3)
+    sc = scenes.make_workload('solids50_4096x64', seed=0, n_rays=4, n_samples=8, n_strokes=3)
     with pytest.raises(RuntimeError, match='CUDA'):
         render.render_strokes(sc['origins'], sc['directions'], sc['viewdirs'], sc['radii'], sc['tdist'],
                               sc['stroke_ids'], sc['shape_params'], sc['color_params'], sc['density_params'])
@@ -103,8 +103,8 @@ def test_product_path_rejects_cpu_tensors():
 
 
 def test_workload_is_deterministic_and_has_the_stated_palette():
-    a = scenes.make_workload('cfg1', seed=3, n_rays=64, n_samples=16)
-    b = scenes.make_workload('cfg1', seed=3, n_rays=64, n_samples=16)
+    a = scenes.make_workload('palette500_65536x128', seed=3, n_rays=64, n_samples=16)
+    b = scenes.make_workload('palette500_65536x128', seed=3, n_rays=64, n_samples=16)
     for k in ('shape_params', 'color_params', 'density_params', 'origins', 'directions', 'tdist'):
         assert torch.equal(a[k], b[k]), k
     types = a['types']
@@ -119,8 +119,8 @@ def test_workload_is_deterministic_and_has_the_stated_palette():
     # every ray points into the 40-degree cone around the camera axis
     cosang = -(a['origins'] * a['directions']).sum(-1) / 4.0
     assert cosang.min() >= math.cos(math.radians(20)) - 1e-6
-    cfg0 = scenes.make_workload('cfg0', seed=0)
-    assert set(cfg0['types']) <= {0, 1, 4} and len(cfg0['types']) == 50
+    solids = scenes.make_workload('solids50_4096x64', seed=0)
+    assert set(solids['types']) <= {0, 1, 4} and len(solids['types']) == 50
 
 
 def test_random_euler_is_the_quaternion_rotation():

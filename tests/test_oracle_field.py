@@ -109,7 +109,7 @@ def test_golden_fixture_reproduces():
     """The committed float64 fixture (written by tests/golden/make_golden.py) matches the oracle."""
     z = np.load(GOLDEN)
     t = {k: torch.from_numpy(z[k]) for k in z.files}
-    for case in ('cfg0', 'cfg1'):
+    for case in ('solids50_4096x64', 'palette500_65536x128'):
         args = [t[f'{case}_{k}'].double() for k in ('origins', 'directions', 'viewdirs', 'radii', 'tdist')]
         out, gs, gc, gd = F.render_field_grads(t[f'{case}_g_rgb'], t[f'{case}_g_acc'], *args,
                                                t[f'{case}_stroke_ids'], t[f'{case}_shape_params'].double(),

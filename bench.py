@@ -385,7 +385,18 @@ def main():
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
+    ap.add_argument('--copy-calibration', type=int, default=0,
+                    help='only copy a device buffer of this many MiB 5 times (HBM counter calibration run)')
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+        src = torch.ones(args.copy_calibration << 18, device='cuda:0')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': src.numel() * 4, 'copies': 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({'error': 'this build measures one GPU; multi-GPU runs are not part of it',
                           'n_gpus': args.gpus}))

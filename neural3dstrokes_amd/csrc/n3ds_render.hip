@@ -1485,19 +1485,30 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     }
 }
 
-// sum per-block partial rows and scatter into the parameter-gradient tensors (+=)
+// Sum per-block partial rows into the parameter-gradient tensors (+=).  blockIdx.y
+// takes a slice of RED_PARTS consecutive rows; slices meet in float atomics (one
+// slice: a plain read-modify-write).
+constexpr int RED_PARTS = 64;
+
+__device__ __forceinline__ void add_grad(float* p, float v, bool atomic) {
+    if (atomic) atomicAdd(p, v);
+    else *p += v;
+}
+
 __global__ void render_reduce_kernel(const float* __restrict__ partials, int nparts, int S, int Pmax, int Q,
                                      float* __restrict__ gshape, float* __restrict__ gcolor,
                                      float* __restrict__ gdens) {
     const int PG = Pmax + Q + 1;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)S * PG) return;
+    const int b0 = blockIdx.y * RED_PARTS, b1 = min(nparts, b0 + RED_PARTS);
     float s = 0.f;
-    for (int b = 0; b < nparts; b++) s += partials[(int64_t)b * S * PG + i];
+    for (int b = b0; b < b1; b++) s += partials[(int64_t)b * S * PG + i];
+    const bool atomic = gridDim.y > 1;
     int k = (int)(i / PG), c = (int)(i - (int64_t)k * PG);
-    if (c < Pmax) gshape[(int64_t)k * Pmax + c] += s;
-    else if (c < Pmax + Q) gcolor[(int64_t)k * Q + (c - Pmax)] += s;
-    else gdens[k] += s;
+    if (c < Pmax) add_grad(&gshape[(int64_t)k * Pmax + c], s, atomic);
+    else if (c < Pmax + Q) add_grad(&gcolor[(int64_t)k * Q + (c - Pmax)], s, atomic);
+    else add_grad(&gdens[k], s, atomic);
 }
 
 // batch of scenes: stroke k of scene s sums the partial rows of s's blocks
@@ -1512,17 +1523,21 @@ __global__ void render_batch_reduce_kernel(const float* __restrict__ partials, c
     if (i >= (int64_t)S_total * PG) return;
     const int k = (int)(i / PG), c = (int)(i - (int64_t)k * PG);
     const int* rec = scenes + find_scene(scenes, n_scenes, 2, k) * SCENE_REC;
+    // blockIdx.y takes a slice of RED_PARTS of the scene's blocks
+    const int b0 = blockIdx.y * RED_PARTS, b1 = min(rec[7] == 0 ? 1 : rec[5], b0 + RED_PARTS);
+    if (b0 >= b1) return;
     float s = 0.f;
     if (rec[7] == 0) {
         if (rec[5] > 0) s = direct[i];
     } else {
         const int64_t row = (int64_t)rec[3] * PG;
         const float* p = partials + rec[6] + (int64_t)(k - rec[2]) * PG + c;
-        for (int b = 0; b < rec[5]; b++) s += p[b * row];
+        for (int b = b0; b < b1; b++) s += p[b * row];
     }
-    if (c < Pmax) gshape[(int64_t)k * Pmax + c] += s;
-    else if (c < Pmax + Q) gcolor[(int64_t)k * Q + (c - Pmax)] += s;
-    else gdens[k] += s;
+    const bool atomic = gridDim.y > 1;
+    if (c < Pmax) add_grad(&gshape[(int64_t)k * Pmax + c], s, atomic);
+    else if (c < Pmax + Q) add_grad(&gcolor[(int64_t)k * Q + (c - Pmax)], s, atomic);
+    else add_grad(&gdens[k], s, atomic);
 }
 
 __global__ void fill_zero_kernel(float* __restrict__ p, int64_t n) {
@@ -1759,7 +1774,10 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * bs.waves), (size_t)bs.lds_bytes, s, A, grad_rgb, grad_acc,
                        grad_depth, state, parts, bs.use_lds, direct);
     N3DS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(render_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace,
+    const int nparts = bs.use_lds ? (int)nb : 1;
+    hipLaunchKernelGGL(render_reduce_kernel,
+                       dim3((unsigned)((tot + 255) / 256), (unsigned)((nparts + RED_PARTS - 1) / RED_PARTS)), dim3(256),
+                       0, s, workspace,
                        bs.use_lds ? (int)nb : 1, n_strokes, shape_width, color_width, grad_shape, grad_color,
                        grad_density);
     N3DS_LAUNCH_CHECK();
@@ -2032,7 +2050,11 @@ N3DS_EXPORT int n3ds_render_batch_backward(const float* origins, const float* di
         N3DS_LAUNCH_CHECK();
     }
     const int64_t tot = (int64_t)n_strokes * PG;
-    hipLaunchKernelGGL(render_batch_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, parts,
+    int max_blocks = 1;
+    for (int i = 0; i < n_scenes; i++) max_blocks = max(max_blocks, (int)scenes_host[(int64_t)i * SCENE_REC + 5]);
+    hipLaunchKernelGGL(render_batch_reduce_kernel,
+                       dim3((unsigned)((tot + 255) / 256), (unsigned)((max_blocks + RED_PARTS - 1) / RED_PARTS)),
+                       dim3(256), 0, s, parts,
                        direct, scenes, n_scenes, n_strokes, shape_width, color_width, grad_shape, grad_color,
                        grad_density);
     N3DS_LAUNCH_CHECK();

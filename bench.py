@@ -346,6 +346,24 @@ def multiscene_block(n_scenes=32, steps=10, warmup=2, seed=0, loop_steps=3):
             'loss': float(trainer.loss)}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                            'render_hbm_traffic_palette500_65536x128.json')
+
+
+def hbm_traffic(workload, kernel):
+    """HBM bytes per launch of a render kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE record (scaled by a plain copy's readings); None
+    when there is no record for the workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get('workload') != workload:
+        return None
+    return rec.get('bytes_per_launch', {}).get(kernel)
+
+
 def cpu_baseline(sc, n_rays, g_rgb, g_acc):
     """Float32 CPU oracle (plain per-op math, no culling) fwd+bwd on the first n_rays rays."""
     import torch
@@ -478,7 +496,7 @@ def main():
                                f'fwd+bwd, {args.adjoint} adjoint', 'strokes': S, 'rays': R, 'samples_per_ray': n,
                    'adjoint': args.adjoint},
         'roofline': {'bound': 'valu', 'kernel': dom, 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
-                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': None,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': hbm_traffic(args.workload, dom),
                      'flops_per_launch': kernels[dom][0], 'ms_per_launch': kernels[dom][1]},
         'kernels_ms': {'render_fwd': fwd_ms, 'render_bwd': bwd_ms},
         'useful_pairs': {BASE_NAMES[b]: c for b, c in pos_pairs.items()},

@@ -11,8 +11,8 @@
 //    p -= lr / (1 - b1^t) * m / (sqrt(v) / sqrt(1 - b2^t) + eps);
 //  * StrokeField.clip_params (source/models.py:576-584): per-column clamp of the
 //    shape / colour parameters to their ranges, density >= 0.
-// Memory-bound elementwise work (28 bytes per parameter); one thread per
-// element, the column bounds in LDS.
+// Memory-bound elementwise work (32 bytes per parameter: p, g, m, v read and
+// written); one thread per element, the column bounds in LDS.
 #include "n3ds_capi.h"
 
 namespace n3ds {

@@ -576,23 +576,30 @@ __device__ __forceinline__ int stroke_class(const float* rec, bool curve) {
     return curve ? (ibits(rec[R_BASE]) == QUAD_BEZIER ? 1 : 0) : 2;
 }
 
-// Lists the hits of class x among the window's hits (lane h holds hit hc = c0 + h, of
-// class hcls; -1 past the window): w.xh[r] = r-th such hit, w.xo[r] = its first pair
-// in the class's pair sequence.  Returns the class's number of pairs; nx = hits.
-__device__ __forceinline__ int class_hits(const WaveMem& w, int hc, int hcls, int x, int lane, int& nx) {
-    const bool in = hcls == x;
-    const unsigned long long m = __ballot(in);
-    nx = __popcll(m);
-    if (!m) return 0;
-    const int cnt = in ? w.ccnt[hc] : 0;
-    const int incl = wave_incl_scan(cnt, lane);
-    if (in) {
-        const int r = __popcll(m & ((1ull << lane) - 1ull));
-        w.xo[r] = incl - cnt;
-        w.xh[r] = hc;
+// Orders the window's hits by class (stable): w.xh[r] = r-th hit of the order,
+// w.xo[r] = its first pair in the class-ordered pair sequence.  Lane h holds hit
+// hc = c0 + h of class hcls (-1 past the window).  Returns the number of hits.
+__device__ __forceinline__ int class_sort_hits(const WaveMem& w, int hc, int hcls, int lane) {
+    const int cnt = hcls >= 0 ? w.ccnt[hc] : 0;
+    int before_hits = 0, before_pairs = 0, rank = 0, xo = 0;
+#pragma unroll
+    for (int x = 0; x < NCLASS; x++) {
+        const bool in = hcls == x;
+        const unsigned long long m = __ballot(in);
+        const int incl = wave_incl_scan(in ? cnt : 0, lane);
+        if (in) {
+            rank = before_hits + __popcll(m & ((1ull << lane) - 1ull));
+            xo = before_pairs + incl - cnt;
+        }
+        before_hits += __popcll(m);
+        before_pairs += __shfl(incl, 63, 64);
+    }
+    if (hcls >= 0) {
+        w.xo[rank] = xo;
+        w.xh[rank] = hc;
     }
     wave_sync();
-    return __shfl(incl, 63, 64);
+    return before_hits;
 }
 
 // r-th hit of the current class list holding class pair pp
@@ -879,12 +886,13 @@ __global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 
             const int wbase = w.coff[c0];
             const int c1 = window_end(w, c0, nh, wbase, lane);
             const int wp = w.coff[c1 - 1] + w.ccnt[c1 - 1] - wbase;
-            // alpha of every (sample, stroke) pair of the window, one pair per lane, class by class
+            // alpha of every (sample, stroke) pair of the window, one pair per lane, in class
+            // order (a 64-pair group spans at most a class boundary or two)
             const int hc = c0 + lane;
             const int hcls = hc < c1 ? (w.ccls[hc] & 3) : -1;
-            for (int xc = 0; xc < NCLASS; xc++) {
-              int nx;
-              const int xt = class_hits(w, hc, hcls, xc, lane, nx);
+            {
+              const int nx = class_sort_hits(w, hc, hcls, lane);
+              const int xt = wp;
               for (int p0 = 0; p0 < xt; p0 += 64) {
                 const int pp = p0 + lane;
                 if (pp < xt) {
@@ -916,7 +924,6 @@ __global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 
                     }
                 }
               }
-              if (xt) wave_sync();
             }
             wave_sync();
             // ordered over-compositing, strokes front to back
@@ -1256,9 +1263,9 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 }
                 const int hc = c0 + lane;
                 const int hcls = hc < c1 ? (w.ccls[hc] & 3) : -1;
-                for (int xc = 0; !gkept && xc < NCLASS; xc++) {
-                  int nx;
-                  const int xt = class_hits(w, hc, hcls, xc, lane, nx);
+                if (!gkept) {
+                  const int nx = class_sort_hits(w, hc, hcls, lane);
+                  const int xt = wp;
                   for (int p0 = 0; p0 < xt; p0 += 64) {
                     const int pp = p0 + lane;
                     if (pp < xt) {
@@ -1282,7 +1289,6 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                         }
                     }
                   }
-                  if (xt) wave_sync();
                 }
                 wave_sync();
                 // (2) compositing adjoint, strokes front to back; per pair it leaves
@@ -1377,10 +1383,9 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                     }
                 }
                 wave_sync();
-                for (int xc = 0; xc < NCLASS; xc++)
-                for (int u0 = xs[xc]; u0 < xs[xc + 1]; u0 += 64) {
+                const int nu = xs[NCLASS];
+                for (int u0 = 0; u0 < nu; u0 += 64) {
                     const int u = u0 + lane;
-                    const int nu = xs[xc + 1];
                     float G[NG];
 #pragma unroll
                     for (int g = 0; g < NG; g++) G[g] = 0.f;
@@ -1445,9 +1450,9 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                             }
                         }
                     }
-                    // segmented sum over the lanes of each hit through an LDS transpose (lanes
-                    // hold nondecreasing hit indices; w.xo lists the first lane of each hit)
-                    const int nl = min(64, xs[xc + 1] - u0);
+                    // segmented sum over the lanes of each hit through an LDS transpose (the
+                    // lanes of one hit are contiguous; w.xo lists the first lane of each run)
+                    const int nl = min(64, nu - u0);
 #pragma unroll
                     for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
                     w.scr[lane * (NG + 1) + NG] = __int_as_float(c);

@@ -170,7 +170,11 @@ int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* sh
  * npf = 4 for color_id 1 (gradient_rgb) else 1.  Per ray, the culled hits
  * and the alpha (and colour) of every evaluated (sample, stroke) pair are
  * kept chunk by chunk (64 strokes) until the ray's capacity is used up; the
- * backward recomputes the chunks that did not fit. */
+ * backward recomputes the chunks that did not fit.
+ * workspace (device, n3ds_render_forward_workspace_bytes; or NULL): when
+ * given, culling runs first as its own pass, one wavefront per (ray, chunk of
+ * 64 strokes), and the forward reads its hits; when NULL the forward culls
+ * as it goes (no extra memory).  Results are identical. */
 int n3ds_render_forward(const float* origins, const float* directions, const float* viewdirs, const float* radii,
                         const float* tdist, const float* table, const float* color_params,
                         const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
@@ -178,7 +182,12 @@ int n3ds_render_forward(const float* origins, const float* directions, const flo
                         int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
                         const float* bg_color, float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
                         float* state_out, unsigned long long* counters, int* hit_dir, int* hit_buf,
-                        float* pair_buf, int cap_hits, int cap_pairs, void* stream);
+                        float* pair_buf, int cap_hits, int cap_pairs, void* workspace, int64_t workspace_bytes,
+                        void* stream);
+
+/* Bytes of the forward's culling-pass workspace: 260 B per (ray, chunk of 64
+ * strokes); max_strokes = the stroke count (a batch: its largest scene's). */
+int64_t n3ds_render_forward_workspace_bytes(int64_t n_rays, int max_strokes);
 
 /* Floats of device workspace n3ds_render_backward needs (queries the current
  * device; -1 on a HIP error). */
@@ -235,7 +244,7 @@ int n3ds_render_batch_forward(const float* origins, const float* directions, con
                               int inv_scale_radius, int cone, float coord_div, const float* bg_color, float cull_eps,
                               float* rgb_out, float* acc_out, float* depth_out, float* state_out,
                               unsigned long long* counters, int* hit_dir, int* hit_buf, float* pair_buf, int cap_hits,
-                              int cap_pairs, void* stream);
+                              int cap_pairs, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Floats of device workspace n3ds_render_batch_backward needs for a planned
  * scene table (-1 on a HIP error). */

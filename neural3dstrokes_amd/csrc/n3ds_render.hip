@@ -98,6 +98,10 @@ struct RenderArgs {
     const int* scenes;
     int n_scenes;
     int block_base;  // backward: global block number of this launch's first block
+    // hits of the culling pass (nullptr: the forward culls): per (ray, chunk) a count
+    // word (hits | pairs << 8) and up to 64 packed hits (pack_hit), index ray * nch_stride + chunk
+    const int* ccounts;
+    const int* cslots;
 };
 
 constexpr int SCENE_REC = 8;
@@ -634,13 +638,14 @@ __device__ __forceinline__ bool stage_depths(const RenderArgs& A, int64_t ray, i
     return __ballot(bad) == 0ull;
 }
 
-// Culls the 64 strokes of the chunk starting at base0 (in compositing order)
-// and appends the compacted hits to the pending list after its nh hits / np
-// pairs.  Returns the number of new hits; tot receives their number of pairs.
+// Culls the 64 strokes of the chunk starting at base0 (in compositing order),
+// one per lane: stroke k, whether it is hit, its samples [i0, i0 + cnt), its
+// class and, for a curve, the mask of its pieces whose chord boxes pass (the
+// samples then cover the union of those boxes, to be narrowed by capsule_range).
 template <int NS>
-__device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
-                                           int base0, int lane, int nh, int np, int& tot) {
-    const int k = A.S - 1 - (base0 + lane);
+__device__ __forceinline__ bool cull_chunk(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
+                                           int base0, int lane, int& k, int& i0, int& cnt, int& cls, int& pmask) {
+    k = A.S - 1 - (base0 + lane);
     float ta = 0.f, tb = -1.f;
     bool curve = false;
     bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi,
@@ -652,7 +657,7 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
     // evaluated (many chunks at once).  w.pb is free outside the evaluation windows.
     static_assert(NPIECE == 16, "one 16-lane group per curve");
     const unsigned long long cm = __ballot(hit && curve);
-    int pmask = 0;
+    pmask = 0;
     if (cm) {
         int* src = (int*)w.pb;      // [64] lane of the c-th curve
         float* res = w.pb + 64;     // [64][2] box-union interval
@@ -699,7 +704,9 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
         }
         wave_sync();
     }
-    int i0 = 0, cnt = 0;
+    i0 = 0;
+    cnt = 0;
+    cls = 2;
     if (hit) {
         if (sorted) {
             i0 = count_below<NS, false>(w.ts, ta);
@@ -708,7 +715,52 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
             cnt = A.n;
         }
         hit = cnt > 0;
+        cls = stroke_class(A.table + (int64_t)k * REC, curve);
     }
+    return hit;
+}
+
+// Samples [i0, i0 + cnt) of curve stroke k within the chord capsules of its pieces in
+// pmask (each piece tested in turn by the calling lane); cnt = 0 if none is hit.
+template <int NS>
+__device__ __forceinline__ void capsule_range(const RenderArgs& A, const RayConst& rc, const float* ts, bool sorted,
+                                              int k, unsigned pmask, int& i0, int& cnt) {
+    const float* rec = A.table + (int64_t)k * REC;
+    const float e0 = rec[R_RMAX] + rc.m;
+    float u0 = 1e30f, u1 = -1e30f;
+    for (unsigned m = pmask; m; m &= m - 1) {
+        const int pc = __builtin_ctz(m);
+        float p0, p1;
+        if (capsule_interval(rc.oc, rc.dc, rc.idd, ld3(rec + R_PTS + 3 * pc), ld3(rec + R_PTS + 3 * pc + 3),
+                             e0 + rec[R_DEV + pc], p0, p1)) {
+            u0 = fminf(u0, fmaxf(p0, rc.tlo));
+            u1 = fmaxf(u1, fminf(p1, rc.thi));
+        }
+    }
+    cnt = 0;
+    if (u0 <= u1) {
+        float ta, tb;
+        pad_interval(u0, u1, ta, tb);
+        if (sorted) {
+            i0 = count_below<NS, false>(ts, ta);
+            cnt = count_below<NS, true>(ts, tb) - i0;
+        } else {
+            i0 = 0;
+            cnt = A.n;
+        }
+    }
+}
+
+// Culls a chunk (cull_chunk) and appends the compacted hits to the pending list
+// after its nh hits / np pairs; curve hits stay provisional (class | 4 | pieces
+// << 16) until refine_pending.  Returns the number of new hits; tot receives
+// their number of pairs.
+template <int NS>
+__device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& rc, const WaveMem& w, bool sorted,
+                                           int base0, int lane, int nh, int np, int& tot) {
+    int k, i0, cnt, cls, pmask;
+    const bool hit = cull_chunk<NS>(A, rc, w, sorted, base0, lane, k, i0, cnt, cls, pmask);
+    const bool curve = cls < 2;
     const unsigned long long mask = __ballot(hit);
     const int incl = wave_incl_scan(hit ? cnt : 0, lane);
     if (hit) {
@@ -718,7 +770,7 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
         w.coff[rank] = np + incl - cnt;
         w.ccnt[rank] = cnt;
         // class | provisional (curve still to refine) << 2 | passing pieces << 16
-        w.ccls[rank] = stroke_class(A.table + (int64_t)k * REC, curve) | (curve ? 4 | (pmask << 16) : 0);
+        w.ccls[rank] = cls | (curve ? 4 | (pmask << 16) : 0);
     }
     tot = __shfl(incl, 63, 64);
     wave_sync();
@@ -760,30 +812,7 @@ __device__ __forceinline__ int refine_pending(const RenderArgs& A, const RayCons
             cls = w.ccls[c];
         }
         if (c < nh && (cls & 4)) {
-            const float* rec = A.table + (int64_t)k * REC;
-            const float e0 = rec[R_RMAX] + rc.m;
-            float u0 = 1e30f, u1 = -1e30f;
-            for (unsigned m = (unsigned)cls >> 16; m; m &= m - 1) {
-                const int pc = __builtin_ctz(m);
-                float p0, p1;
-                if (capsule_interval(rc.oc, rc.dc, rc.idd, ld3(rec + R_PTS + 3 * pc), ld3(rec + R_PTS + 3 * pc + 3),
-                                     e0 + rec[R_DEV + pc], p0, p1)) {
-                    u0 = fminf(u0, fmaxf(p0, rc.tlo));
-                    u1 = fmaxf(u1, fminf(p1, rc.thi));
-                }
-            }
-            cnt = 0;
-            if (u0 <= u1) {
-                float ta, tb;
-                pad_interval(u0, u1, ta, tb);
-                if (sorted) {
-                    i0 = count_below<NS, false>(w.ts, ta);
-                    cnt = count_below<NS, true>(w.ts, tb) - i0;
-                } else {
-                    i0 = 0;
-                    cnt = A.n;
-                }
-            }
+            capsule_range<NS>(A, rc, w.ts, sorted, k, (unsigned)cls >> 16, i0, cnt);
             cls &= 3;
         }
         const bool keep = c < nh && cnt > 0;
@@ -804,6 +833,58 @@ __device__ __forceinline__ int refine_pending(const RenderArgs& A, const RayCons
     }
     np = pairs;
     return out;
+}
+
+// ---------------------------------------------------------------------------
+// culling pass: one wavefront per (ray, chunk of 64 strokes)
+// ---------------------------------------------------------------------------
+constexpr int CWAVES = 4;  // wavefronts per culling block
+
+__host__ __device__ constexpr int cull_wave_floats(int ns) { return 64 * ns + 256; }
+
+// lane in chunk | first sample << 6 | samples << 15 | class << 24 (samples <= 256)
+__device__ __forceinline__ int pack_hit(int l, int i0, int cnt, int cls) {
+    return l | (i0 << 6) | (cnt << 15) | (cls << 24);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0, int* __restrict__ counts,
+                                                                  int* __restrict__ slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t t = (int64_t)blockIdx.x * CWAVES + wave;
+    if (t >= A0.R * (int64_t)A0.nch_stride) return;
+    // chunk-major: neighbouring wavefronts cull the same strokes for different rays
+    const int ch = (int)(t / A0.R);
+    const int64_t ray = t - (int64_t)ch * A0.R;
+    RenderArgs A = A0;
+    if (A0.scenes) {
+        const int sc = find_scene(A0.scenes, A0.n_scenes, 0, __builtin_amdgcn_readfirstlane((int)ray));
+        scene_strokes(A, A0.scenes + sc * SCENE_REC);
+    }
+    const int64_t slot = ray * A0.nch_stride + ch;
+    if (ch >= A.nch) {
+        if (lane == 0) counts[slot] = 0;
+        return;
+    }
+    WaveMem w;
+    w.ts = smem + wave * cull_wave_floats(NS);
+    w.pb = w.ts + 64 * NS;
+    const RayConst rc = load_ray(A, ray);
+    float ts[NS];
+    bool valid[NS];
+    const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
+    int k, i0, cnt, cls, pmask;
+    bool hit = cull_chunk<NS>(A, rc, w, sorted, ch * 64, lane, k, i0, cnt, cls, pmask);
+    if (hit && cls < 2) {
+        capsule_range<NS>(A, rc, w.ts, sorted, k, (unsigned)pmask, i0, cnt);
+        hit = cnt > 0;
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int incl = wave_incl_scan(hit ? cnt : 0, lane);
+    if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);
+    const int tot = __shfl(incl, 63, 64);
+    if (lane == 0) counts[slot] = __popcll(mask) | (tot << 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -843,13 +924,36 @@ __global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 
     const int64_t rp = ray * (int64_t)A.cap_p * NPF;
     int nh = 0, np = 0;  // pending hits / pairs
     int gch = 0;         // first chunk of the pending group
+    int cword = 0;       // culling pass: count word of chunk (ch & ~63) + lane
     for (int ch = 0; ch < A.nch; ch++) {
-        int tot;
-        const int nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
+        int nc, tot;
+        if (A.ccounts) {
+            // hits of the culling pass, final (capsules applied)
+            const int64_t slot = ray * A.nch_stride + ch;
+            if ((ch & 63) == 0) cword = ch + lane < A.nch ? A.ccounts[slot + lane] : 0;
+            const int cw = __builtin_amdgcn_readlane(cword, ch & 63);
+            nc = cw & 127;
+            tot = cw >> 8;
+            if (nc) {
+                const int hw = lane < nc ? A.cslots[slot * 64 + lane] : 0;
+                const int cnt = lane < nc ? (hw >> 15) & 511 : 0;
+                const int incl = wave_incl_scan(cnt, lane);
+                if (lane < nc) {
+                    w.ck[nh + lane] = A.S - 1 - (ch * 64 + (hw & 63));
+                    w.ci0[nh + lane] = (hw >> 6) & 511;
+                    w.ccnt[nh + lane] = cnt;
+                    w.ccls[nh + lane] = (hw >> 24) & 3;
+                    w.coff[nh + lane] = np + incl - cnt;
+                }
+                wave_sync();
+            }
+        } else {
+            nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
+        }
         nh += nc;
         np += tot;
         if (nh <= HMAX - 64 && np < FLUSH_PAIRS && ch + 1 < A.nch) continue;
-        nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
+        if (!A.ccounts) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
         int kept_np = 0;
         const int group_poff = poff;
         if (A.hdir) {
@@ -1656,6 +1760,8 @@ static RenderArgs make_args(const float* origins, const float* dirs, const float
     A.scenes = nullptr;
     A.n_scenes = 0;
     A.block_base = 0;
+    A.ccounts = nullptr;
+    A.cslots = nullptr;
     return A;
 }
 
@@ -1672,6 +1778,40 @@ static int set_keep(RenderArgs& A, int* hit_dir, int* hit_buf, float* pair_buf, 
 }
 
 N3DS_EXPORT int64_t n3ds_render_table_floats(int n_strokes) { return (int64_t)n_strokes * REC; }
+
+typedef void (*CullKernel)(RenderArgs, int*, int*);
+
+static CullKernel cull_kernel_ptr(int ns) {
+    static const CullKernel tab[4] = {render_cull_kernel<1>, render_cull_kernel<2>, render_cull_kernel<3>,
+                                      render_cull_kernel<4>};
+    return tab[ns - 1];
+}
+
+static int64_t cull_workspace_bytes(int64_t n_rays, int nch) { return n_rays * nch * 65 * 4; }
+
+// Runs the culling pass into the caller's workspace (if given) and points A at its hits.
+static int launch_cull(RenderArgs& A, int ns, void* workspace, int64_t workspace_bytes, hipStream_t s) {
+    if (!workspace) return N3DS_OK;
+    const int64_t ntask = A.R * A.nch_stride;
+    N3DS_REQUIRE(workspace_bytes >= cull_workspace_bytes(A.R, A.nch_stride),
+                 "forward workspace has %lld bytes, need %lld", (long long)workspace_bytes,
+                 (long long)cull_workspace_bytes(A.R, A.nch_stride));
+    N3DS_REQUIRE(ntask / CWAVES < (1ll << 31), "too many (ray, chunk) culling tasks");
+    if (ntask == 0) return N3DS_OK;
+    int* counts = (int*)workspace;
+    int* slots = counts + ntask;
+    hipLaunchKernelGGL(cull_kernel_ptr(ns), dim3((unsigned)((ntask + CWAVES - 1) / CWAVES)), dim3(64 * CWAVES),
+                       CWAVES * cull_wave_floats(ns) * 4, s, A, counts, slots);
+    N3DS_LAUNCH_CHECK();
+    A.ccounts = counts;
+    A.cslots = slots;
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int64_t n3ds_render_forward_workspace_bytes(int64_t n_rays, int max_strokes) {
+    if (n_rays < 0 || max_strokes < 0) return -1;
+    return cull_workspace_bytes(n_rays, (max_strokes + 63) / 64);
+}
 
 N3DS_EXPORT int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* shape_params,
                                     int n_strokes, int shape_width, int* status, void* stream) {
@@ -1691,7 +1831,8 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
                                     float coord_div, const float* bg_color, float cull_eps, float* rgb_out,
                                     float* acc_out, float* depth_out, float* state_out,
                                     unsigned long long* counters, int* hit_dir, int* hit_buf, float* pair_buf,
-                                    int cap_hits, int cap_pairs, void* stream) {
+                                    int cap_hits, int cap_pairs, void* workspace, int64_t workspace_bytes,
+                                    void* stream) {
     int st = render_check(n_rays, n_samples, n_strokes, shape_width, color_width, color_id);
     if (st) return st;
     if (n_rays == 0) return N3DS_OK;
@@ -1702,6 +1843,8 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
+    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream);
+    if (st) return st;
     hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
                        A, rgb_out, acc_out, depth_out, state_out, counters);
     N3DS_LAUNCH_CHECK();
@@ -1914,7 +2057,7 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
                                           float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
                                           float* state_out, unsigned long long* counters, int* hit_dir,
                                           int* hit_buf, float* pair_buf, int cap_hits, int cap_pairs,
-                                          void* stream) {
+                                          void* workspace, int64_t workspace_bytes, void* stream) {
     int s_max = 0;
     int st = batch_check(scenes_host, n_scenes, n_rays, n_strokes, &s_max);
     if (st) return st;
@@ -1931,6 +2074,8 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
+    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream);
+    if (st) return st;
     hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
                        A, rgb_out, acc_out, depth_out, state_out, counters);
     N3DS_LAUNCH_CHECK();

@@ -98,13 +98,20 @@ class _render_fn(Function):
             fwd, batch_args = _lib.lib().n3ds_render_forward, ()
         else:
             fwd, batch_args = _lib.lib().n3ds_render_batch_forward, scenes.c_args()
+        # culling as its own pass (260 B per (ray, chunk of 64 strokes)) unless the caller
+        # asked for the low-memory mode
+        cws = None
+        if store is not False:
+            nbytes = int(_lib.lib().n3ds_render_forward_workspace_bytes(R, s_keep))
+            cws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
         _lib.check(fwd(
             _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
             _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax,
             color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
             _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), float(cull_eps), _lib.ptr(rgb), _lib.ptr(acc),
             _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.ptr(hd), _lib.ptr(hb), _lib.ptr(pb), cap_h,
-            cap_p, _lib.stream()))
+            cap_p, _lib.ptr(cws), 0 if cws is None else int(cws.numel()), _lib.stream()))
+        del cws
         empty = torch.empty(0, device=dev)
         ctx.save_for_backward(origins, directions, viewdirs, radii, tdist, table, shape_params, color_params,
                               density_params, state, hd if hd is not None else empty,

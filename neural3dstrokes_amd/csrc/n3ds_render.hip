@@ -854,9 +854,9 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t t = (int64_t)blockIdx.x * CWAVES + wave;
     if (t >= A0.R * (int64_t)A0.nch_stride) return;
-    // chunk-major: neighbouring wavefronts cull the same strokes for different rays
-    const int ch = (int)(t / A0.R);
-    const int64_t ray = t - (int64_t)ch * A0.R;
+    // ray-major: neighbouring wavefronts share the ray's depths and constants
+    const int64_t ray = t / A0.nch_stride;
+    const int ch = (int)(t - ray * A0.nch_stride);
     RenderArgs A = A0;
     if (A0.scenes) {
         const int sc = find_scene(A0.scenes, A0.n_scenes, 0, __builtin_amdgcn_readfirstlane((int)ray));
@@ -890,12 +890,13 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-template <int NS, int CID>
-__global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 8))) void render_fwd_kernel(RenderArgs A0, float* __restrict__ rgb_out,
-                                                                 float* __restrict__ acc_out,
-                                                                 float* __restrict__ depth_out,
-                                                                 float* __restrict__ state,
-                                                                 unsigned long long* __restrict__ counters) {
+// PRE: the hits come from the culling pass (render_cull_kernel); otherwise the
+// forward culls chunk by chunk itself.
+template <int NS, int CID, bool PRE>
+__device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __restrict__ rgb_out,
+                                                float* __restrict__ acc_out, float* __restrict__ depth_out,
+                                                float* __restrict__ state,
+                                                unsigned long long* __restrict__ counters) {
     extern __shared__ float smem[];
     constexpr int NPF = CID == 1 ? 4 : 1;  // alpha (+ colour for gradient_rgb)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -927,7 +928,7 @@ __global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 
     int cword = 0;       // culling pass: count word of chunk (ch & ~63) + lane
     for (int ch = 0; ch < A.nch; ch++) {
         int nc, tot;
-        if (A.ccounts) {
+        if constexpr (PRE) {
             // hits of the culling pass, final (capsules applied)
             const int64_t slot = ray * A.nch_stride + ch;
             if ((ch & 63) == 0) cword = ch + lane < A.nch ? A.ccounts[slot + lane] : 0;
@@ -953,7 +954,7 @@ __global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 
         nh += nc;
         np += tot;
         if (nh <= HMAX - 64 && np < FLUSH_PAIRS && ch + 1 < A.nch) continue;
-        if (!A.ccounts) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
+        if constexpr (!PRE) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
         int kept_np = 0;
         const int group_poff = poff;
         if (A.hdir) {
@@ -1168,6 +1169,20 @@ __device__ __forceinline__ int slot_column(int slot, const float* __restrict__ r
         return off < 0 ? -1 : off + slot - 8;
     }
     return -1;
+}
+
+template <int NS, int CID>
+__global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(3, 8))) void render_fwd_kernel(
+    RenderArgs A0, float* __restrict__ rgb_out, float* __restrict__ acc_out, float* __restrict__ depth_out,
+    float* __restrict__ state, unsigned long long* __restrict__ counters) {
+    render_fwd_body<NS, CID, false>(A0, rgb_out, acc_out, depth_out, state, counters);
+}
+
+template <int NS, int CID>
+__global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(4, 8))) void render_fwd_pre_kernel(
+    RenderArgs A0, float* __restrict__ rgb_out, float* __restrict__ acc_out, float* __restrict__ depth_out,
+    float* __restrict__ state, unsigned long long* __restrict__ counters) {
+    render_fwd_body<NS, CID, true>(A0, rgb_out, acc_out, depth_out, state, counters);
 }
 
 template <int NS, int CID>
@@ -1660,6 +1675,19 @@ static int g_max_lds = 0;
 typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
 typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
 
+static FwdKernel fwd_pre_kernel_ptr(int ns, int cid) {
+    static const FwdKernel tab[4][4] = {
+        {render_fwd_pre_kernel<1, 0>, render_fwd_pre_kernel<1, 1>, render_fwd_pre_kernel<1, 2>,
+         render_fwd_pre_kernel<1, 3>},
+        {render_fwd_pre_kernel<2, 0>, render_fwd_pre_kernel<2, 1>, render_fwd_pre_kernel<2, 2>,
+         render_fwd_pre_kernel<2, 3>},
+        {render_fwd_pre_kernel<3, 0>, render_fwd_pre_kernel<3, 1>, render_fwd_pre_kernel<3, 2>,
+         render_fwd_pre_kernel<3, 3>},
+        {render_fwd_pre_kernel<4, 0>, render_fwd_pre_kernel<4, 1>, render_fwd_pre_kernel<4, 2>,
+         render_fwd_pre_kernel<4, 3>}};
+    return tab[ns - 1][cid];
+}
+
 static FwdKernel fwd_kernel_ptr(int ns, int cid) {
     static const FwdKernel tab[4][4] = {
         {render_fwd_kernel<1, 0>, render_fwd_kernel<1, 1>, render_fwd_kernel<1, 2>, render_fwd_kernel<1, 3>},
@@ -1845,7 +1873,8 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     const int ns = (n_samples + 63) / 64;
     st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream);
     if (st) return st;
-    hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
+    hipLaunchKernelGGL(A.ccounts ? fwd_pre_kernel_ptr(ns, color_id) : fwd_kernel_ptr(ns, color_id), grid, block,
+                       fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
                        A, rgb_out, acc_out, depth_out, state_out, counters);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;
@@ -2076,7 +2105,8 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
     const int ns = (n_samples + 63) / 64;
     st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream);
     if (st) return st;
-    hipLaunchKernelGGL(fwd_kernel_ptr(ns, color_id), grid, block, fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
+    hipLaunchKernelGGL(A.ccounts ? fwd_pre_kernel_ptr(ns, color_id) : fwd_kernel_ptr(ns, color_id), grid, block,
+                       fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
                        A, rgb_out, acc_out, depth_out, state_out, counters);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;

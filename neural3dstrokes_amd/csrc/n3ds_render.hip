@@ -838,6 +838,80 @@ __device__ __forceinline__ int refine_pending(const RenderArgs& A, const RayCons
 // ---------------------------------------------------------------------------
 // culling pass: one wavefront per (ray, chunk of 64 strokes)
 // ---------------------------------------------------------------------------
+
+// cull_chunk with final curve intervals: every piece of a hull-hit curve is tested
+// against its chord capsule directly, one lane per piece (no box prefilter: the 16
+// lanes of a curve run the capsule math anyway), and the union of the hit
+// capsules is the curve's interval.
+template <int NS>
+__device__ __forceinline__ bool cull_chunk_final(const RenderArgs& A, const RayConst& rc, const WaveMem& w,
+                                                 bool sorted, int base0, int lane, int& k, int& i0, int& cnt,
+                                                 int& cls) {
+    k = A.S - 1 - (base0 + lane);
+    float ta = 0.f, tb = -1.f;
+    bool curve = false;
+    bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr, rc.tlo, rc.thi,
+                                     ta, tb, curve);
+    const unsigned long long cm = __ballot(hit && curve);
+    if (cm) {
+        int* src = (int*)w.pb;   // [64] lane of the c-th curve
+        float* res = w.pb + 64;  // [64][2] curve interval
+        const int crank = __popcll(cm & ((1ull << lane) - 1ull));
+        if (hit && curve) src[crank] = lane;
+        wave_sync();
+        const int ncv = __popcll(cm);
+        const int kb = A.S - 1 - base0;
+        for (int q0 = 0; q0 < ncv; q0 += 64 / NPIECE) {
+            const int cs = q0 + lane / NPIECE;
+            const int piece = lane % NPIECE;
+            float u0 = 1e30f, u1 = -1e30f;
+            if (cs < ncv) {
+                const float* rec = A.table + (int64_t)(kb - src[cs]) * REC;
+                float p0, p1;
+                if (capsule_interval(rc.oc, rc.dc, rc.idd, ld3(rec + R_PTS + 3 * piece),
+                                     ld3(rec + R_PTS + 3 * piece + 3), rec[R_RMAX] + rc.m + rec[R_DEV + piece], p0,
+                                     p1)) {
+                    p0 = fmaxf(p0, rc.tlo);
+                    p1 = fminf(p1, rc.thi);
+                    if (p0 <= p1) {
+                        u0 = p0;
+                        u1 = p1;
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = NPIECE / 2; off > 0; off >>= 1) {
+                u0 = fminf(u0, __shfl_xor(u0, off, 64));
+                u1 = fmaxf(u1, __shfl_xor(u1, off, 64));
+            }
+            if (cs < ncv && piece == 0) {
+                res[2 * cs] = u0;
+                res[2 * cs + 1] = u1;
+            }
+        }
+        wave_sync();
+        if (hit && curve) {
+            const float u0 = res[2 * crank], u1 = res[2 * crank + 1];
+            hit = u0 <= u1;
+            if (hit) pad_interval(u0, u1, ta, tb);
+        }
+        wave_sync();
+    }
+    i0 = 0;
+    cnt = 0;
+    cls = 2;
+    if (hit) {
+        if (sorted) {
+            i0 = count_below<NS, false>(w.ts, ta);
+            cnt = count_below<NS, true>(w.ts, tb) - i0;
+        } else {
+            cnt = A.n;
+        }
+        hit = cnt > 0;
+        cls = stroke_class(A.table + (int64_t)k * REC, curve);
+    }
+    return hit;
+}
 constexpr int CWAVES = 4;  // wavefronts per culling block
 
 __host__ __device__ constexpr int cull_wave_floats(int ns) { return 64 * ns + 256; }
@@ -874,12 +948,8 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     float ts[NS];
     bool valid[NS];
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
-    int k, i0, cnt, cls, pmask;
-    bool hit = cull_chunk<NS>(A, rc, w, sorted, ch * 64, lane, k, i0, cnt, cls, pmask);
-    if (hit && cls < 2) {
-        capsule_range<NS>(A, rc, w.ts, sorted, k, (unsigned)pmask, i0, cnt);
-        hit = cnt > 0;
-    }
+    int k, i0, cnt, cls;
+    const bool hit = cull_chunk_final<NS>(A, rc, w, sorted, ch * 64, lane, k, i0, cnt, cls);
     const unsigned long long mask = __ballot(hit);
     const int incl = wave_incl_scan(hit ? cnt : 0, lane);
     if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);

@@ -299,6 +299,28 @@ def multiscene_block(n_scenes=32, steps=10, warmup=2, seed=0, loop_steps=3):
     targets = torch.rand(batch.n_rays, 3, generator=g).to(dev)
     trainer = MultiSceneTrainer(batch, packed, packed['stroke_ids'], packed['shape_params'], packed['color_params'],
                                 packed['density_params'], targets, lr=1e-3)
+    # autotune the batch's launch shapes on the eager step (the plan depends on batch_oversub)
+    from neural3dstrokes_amd import autotune
+    saved = autotune.current()
+    best_ms, best = None, None
+    for bo in autotune.SPACE['batch_oversub']:
+        autotune.set('batch_oversub', bo)
+        batch.replan()
+        for fp in (96, 128, 192):
+            autotune.set('flush_pairs', fp)
+            trainer.eager_step()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            trainer.eager_step()
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1)
+            if best_ms is None or t < best_ms:
+                best_ms, best = t, {'batch_oversub': bo, 'flush_pairs': fp}
+    for k, v in best.items():
+        autotune.set(k, v)
+    batch.replan()
     trainer.capture(warmup=warmup)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     trainer.step()
@@ -338,12 +360,14 @@ def multiscene_block(n_scenes=32, steps=10, warmup=2, seed=0, loop_steps=3):
     torch.cuda.synchronize()
     ms_loop = e0.elapsed_time(e1) / loop_steps
     rs_total = batch.n_rays * n
+    for k, v in saved.items():
+        autotune.set(k, v)
     return {'workload': f'{n_scenes} scenes, strokes log-U[20,1000] (sum {batch.n_strokes}, max {batch.s_max}), '
                         f'rays log-U[1024,65536] (sum {batch.n_rays}) x {n} samples, Adam lr 1e-3',
             'graph_step_ms': ms_graph, 'graph_ray_samples_per_s': rs_total / (ms_graph * 1e-3),
             'per_scene_loop_ms': ms_loop, 'per_scene_loop_ray_samples_per_s': rs_total / (ms_loop * 1e-3),
             'speedup_vs_loop': ms_loop / ms_graph, 'backward_blocks': batch.n_blocks,
-            'loss': float(trainer.loss)}
+            'tuning': best, 'loss': float(trainer.loss)}
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
@@ -403,6 +427,7 @@ def main():
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
+    ap.add_argument('--autotune', type=int, default=1, help='tune launch shapes on the workload first (0: defaults)')
     ap.add_argument('--copy-calibration', type=int, default=0,
                     help='only copy a device buffer of this many MiB 5 times (HBM counter calibration run)')
     args = ap.parse_args()
@@ -455,6 +480,11 @@ def main():
     for _ in range(args.warmup):
         fused()
     torch.cuda.synchronize()
+    # launch shapes chosen by the autotuner on this workload (untimed)
+    tuning, tuning_ms = None, None
+    if args.autotune:
+        from neural3dstrokes_amd import autotune
+        tuning, tuning_ms = autotune.tune(fused, names=('flush_pairs', 'bwd_oversub'))
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
@@ -471,7 +501,8 @@ def main():
     bwd_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
     value = R * n / (ms * 1e-3)
     if args.fused_only:
-        print(json.dumps({'metric': METRIC, 'value': value, 'ms_per_step': ms, 'fwd_ms': fwd_ms, 'bwd_ms': bwd_ms}))
+        print(json.dumps({'metric': METRIC, 'value': value, 'ms_per_step': ms, 'fwd_ms': fwd_ms, 'bwd_ms': bwd_ms,
+                          'tuning': tuning}))
         return
 
     # useful-pair counts (alpha > 0) per stroke type, from an untimed counting pass
@@ -505,6 +536,7 @@ def main():
         'stroke_hits': cnt[24],
         'useful_gflop_per_step': useful_total / 1e9,
         'wall_s_timed': wall,
+        'tuning': {'chosen': tuning, 'candidates_ms': tuning_ms},
     }
 
     if args.plain_rays > 0:

@@ -185,6 +185,16 @@ int n3ds_render_forward(const float* origins, const float* directions, const flo
                         float* pair_buf, int cap_hits, int cap_pairs, void* workspace, int64_t workspace_bytes,
                         void* stream);
 
+/* Launch-shape parameters of the render kernels (no counterpart in the
+ * reference), set by the autotuner for the current process: 0 = pending
+ * (sample, stroke) pairs that trigger an evaluation window (16..256, default
+ * 128), 1 = backward persistent blocks per resident block (1..8, default 1),
+ * 2 = batch backward blocks per resident block (1..64, default 8; read by
+ * n3ds_render_batch_plan).  value < 0 only queries.  Returns the previous
+ * value, or -1 for an unknown parameter or a value out of range.  Results do
+ * not depend on these parameters beyond float summation order. */
+int n3ds_render_tuning(int param, int value);
+
 /* Bytes of the forward's culling-pass workspace: 260 B per (ray, chunk of 64
  * strokes); max_strokes = the stroke count (a batch: its largest scene's). */
 int64_t n3ds_render_forward_workspace_bytes(int64_t n_rays, int max_strokes);
@@ -216,16 +226,17 @@ int n3ds_render_backward(const float* origins, const float* directions, const fl
  * stroke and ray counts render in one launch per stage.  Rays and strokes are
  * packed scene after scene in the same per-ray / per-stroke arrays as above
  * (the stroke table is n3ds_render_prepare over all packed strokes).  The
- * scene table is int32: [n_scenes, 8] records {first ray, rays, first stroke,
+ * scene table is int32: [n_scenes, 10] records {first ray, rays, first stroke,
  * strokes, first backward block, backward blocks, offset of the scene's
  * partial gradient rows, backward wavefronts per block (0: accumulators in
- * global memory)} followed by the scene of every backward block.  It is passed
+ * global memory), first culling task (the scene's rays x its own 64-stroke
+ * chunks, scene after scene), 0} followed by the scene of every backward block.  It is passed
  * twice, in host memory (validated and used to size launches) and as a device
  * copy (read by the kernels).  Results per scene equal n3ds_render_forward /
  * _backward on that scene alone.
  * ------------------------------------------------------------------------ */
 
-/* Fills record fields 4-7 (the backward plan: blocks per scene in proportion
+/* Fills record fields 4-9 (the backward plan: blocks per scene in proportion
  * to each scene's rays x strokes, oversubscribing the device so the hardware
  * dispatcher balances uneven scenes) from fields 0-3, and the block map when
  * block_map holds block_map_cap >= the block count; returns the number of

@@ -43,6 +43,7 @@ SIGNATURES = {
     'n3ds_render_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
                                    _I32, _F, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P]),
     'n3ds_render_forward_workspace_bytes': (_I64, [_I64, _I32]),
+    'n3ds_render_tuning': (_I32, [_I32, _I32]),
     'n3ds_render_backward_workspace_floats': (_I64, [_I64, _I32, _I32, _I32]),
     'n3ds_render_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
                                     _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P,

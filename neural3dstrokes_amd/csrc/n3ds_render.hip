@@ -93,8 +93,8 @@ struct RenderArgs {
     // batch of independent scenes (nullptr: one scene).  Per scene SCENE_REC ints
     // {first ray, rays, first stroke, strokes, first backward block, backward blocks,
     // offset of its partial gradient rows, backward wavefronts per block (0: global
-    // accumulators)}, then the scene of every backward block; rays and strokes of one
-    // scene are contiguous in every per-ray / per-stroke array.
+    // accumulators), first culling task, 0}, then the scene of every backward block;
+    // rays and strokes of one scene are contiguous in every per-ray / per-stroke array.
     const int* scenes;
     int n_scenes;
     int block_base;  // backward: global block number of this launch's first block
@@ -102,9 +102,11 @@ struct RenderArgs {
     // word (hits | pairs << 8) and up to 64 packed hits (pack_hit), index ray * nch_stride + chunk
     const int* ccounts;
     const int* cslots;
+    int64_t ntask;    // culling tasks: rays x chunks (a batch: each scene's own chunks)
+    int flush_pairs;  // pending pairs that trigger evaluation
 };
 
-constexpr int SCENE_REC = 8;
+constexpr int SCENE_REC = 10;
 
 // last scene whose field f is <= v (scenes with nothing in that field share their
 // start with the next scene, so the search lands on the one that owns v)
@@ -509,7 +511,7 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 // one-pair-per-lane batches are full even though a chunk typically covers
 // only a few (sample, stroke) pairs of a ray.
 constexpr int HMAX = 128;         // pending hits per wavefront
-constexpr int FLUSH_PAIRS = 128;  // pending pairs that trigger evaluation
+constexpr int FLUSH_PAIRS = 128;  // pending pairs that trigger evaluation (default; tunable)
 
 // per-wavefront LDS region
 struct WaveMem {
@@ -927,20 +929,24 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t t = (int64_t)blockIdx.x * CWAVES + wave;
-    if (t >= A0.R * (int64_t)A0.nch_stride) return;
+    if (t >= A0.ntask) return;
     // ray-major: neighbouring wavefronts share the ray's depths and constants
-    const int64_t ray = t / A0.nch_stride;
-    const int ch = (int)(t - ray * A0.nch_stride);
     RenderArgs A = A0;
-    if (A0.scenes) {
-        const int sc = find_scene(A0.scenes, A0.n_scenes, 0, __builtin_amdgcn_readfirstlane((int)ray));
-        scene_strokes(A, A0.scenes + sc * SCENE_REC);
+    int64_t ray;
+    int ch;
+    if (A0.scenes) {  // a batch: tasks cover each scene's rays x its own chunks
+        const int* rec = A0.scenes + find_scene(A0.scenes, A0.n_scenes, 8, __builtin_amdgcn_readfirstlane((int)t)) *
+                                         SCENE_REC;
+        scene_strokes(A, rec);
+        if (A.nch == 0) return;
+        const int local = (int)(t - rec[8]);
+        ray = rec[0] + local / A.nch;
+        ch = local - (local / A.nch) * A.nch;
+    } else {
+        ray = t / A0.nch_stride;
+        ch = (int)(t - ray * A0.nch_stride);
     }
     const int64_t slot = ray * A0.nch_stride + ch;
-    if (ch >= A.nch) {
-        if (lane == 0) counts[slot] = 0;
-        return;
-    }
     WaveMem w;
     w.ts = smem + wave * cull_wave_floats(NS);
     w.pb = w.ts + 64 * NS;
@@ -1023,7 +1029,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
         }
         nh += nc;
         np += tot;
-        if (nh <= HMAX - 64 && np < FLUSH_PAIRS && ch + 1 < A.nch) continue;
+        if (nh <= HMAX - 64 && np < A.flush_pairs && ch + 1 < A.nch) continue;
         if constexpr (!PRE) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
         int kept_np = 0;
         const int group_poff = poff;
@@ -1429,7 +1435,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 nh += nc;
                 np += tot;
                 ch++;
-                do_flush = nh > HMAX - 64 || np >= FLUSH_PAIRS || ch == A.nch;
+                do_flush = nh > HMAX - 64 || np >= A.flush_pairs || ch == A.nch;
             }
             if (!do_flush) continue;
             if (!gkept) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
@@ -1742,6 +1748,14 @@ __global__ void fill_zero_kernel(float* __restrict__ p, int64_t n) {
 static int g_num_cu = 0;
 static int g_max_lds = 0;
 
+// Launch-shape parameters chosen by the autotuner (n3ds_render_tuning): pending pairs
+// that trigger an evaluation window, backward persistent blocks per resident block,
+// and the batch backward's blocks per resident block.
+enum { TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_COUNT = 3 };
+static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8};
+static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1};
+static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64};
+
 typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
 typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
 
@@ -1860,6 +1874,7 @@ static RenderArgs make_args(const float* origins, const float* dirs, const float
     A.block_base = 0;
     A.ccounts = nullptr;
     A.cslots = nullptr;
+    A.flush_pairs = g_tune[TUNE_FLUSH_PAIRS];
     return A;
 }
 
@@ -1877,6 +1892,22 @@ static int set_keep(RenderArgs& A, int* hit_dir, int* hit_buf, float* pair_buf, 
 
 N3DS_EXPORT int64_t n3ds_render_table_floats(int n_strokes) { return (int64_t)n_strokes * REC; }
 
+N3DS_EXPORT int n3ds_render_tuning(int param, int value) {
+    if (param < 0 || param >= TUNE_COUNT) {
+        set_error("unknown tuning parameter %d", param);
+        return -1;
+    }
+    const int old = g_tune[param];
+    if (value >= 0) {
+        if (value < g_tune_lo[param] || value > g_tune_hi[param]) {
+            set_error("tuning parameter %d: %d outside [%d, %d]", param, value, g_tune_lo[param], g_tune_hi[param]);
+            return -1;
+        }
+        g_tune[param] = value;
+    }
+    return old;
+}
+
 typedef void (*CullKernel)(RenderArgs, int*, int*);
 
 static CullKernel cull_kernel_ptr(int ns) {
@@ -1888,16 +1919,17 @@ static CullKernel cull_kernel_ptr(int ns) {
 static int64_t cull_workspace_bytes(int64_t n_rays, int nch) { return n_rays * nch * 65 * 4; }
 
 // Runs the culling pass into the caller's workspace (if given) and points A at its hits.
-static int launch_cull(RenderArgs& A, int ns, void* workspace, int64_t workspace_bytes, hipStream_t s) {
+static int launch_cull(RenderArgs& A, int ns, void* workspace, int64_t workspace_bytes, hipStream_t s,
+                       int64_t ntask) {
     if (!workspace) return N3DS_OK;
-    const int64_t ntask = A.R * A.nch_stride;
+    A.ntask = ntask;
     N3DS_REQUIRE(workspace_bytes >= cull_workspace_bytes(A.R, A.nch_stride),
                  "forward workspace has %lld bytes, need %lld", (long long)workspace_bytes,
                  (long long)cull_workspace_bytes(A.R, A.nch_stride));
     N3DS_REQUIRE(ntask / CWAVES < (1ll << 31), "too many (ray, chunk) culling tasks");
     if (ntask == 0) return N3DS_OK;
     int* counts = (int*)workspace;
-    int* slots = counts + ntask;
+    int* slots = counts + A.R * A.nch_stride;
     hipLaunchKernelGGL(cull_kernel_ptr(ns), dim3((unsigned)((ntask + CWAVES - 1) / CWAVES)), dim3(64 * CWAVES),
                        CWAVES * cull_wave_floats(ns) * 4, s, A, counts, slots);
     N3DS_LAUNCH_CHECK();
@@ -1941,7 +1973,7 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
-    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream);
+    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream, n_rays * (int64_t)A.nch_stride);
     if (st) return st;
     hipLaunchKernelGGL(A.ccounts ? fwd_pre_kernel_ptr(ns, color_id) : fwd_kernel_ptr(ns, color_id), grid, block,
                        fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
@@ -2004,7 +2036,7 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
         N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * bs.waves,
                                                                     (size_t)bs.lds_bytes));
         if (per_cu < 1) per_cu = 1;
-        const int64_t fit = (int64_t)g_num_cu * per_cu;
+        const int64_t fit = (int64_t)g_num_cu * per_cu * g_tune[TUNE_BWD_OVERSUB];
         if (fit < nb) nb = fit;
     }
     const int64_t tot = (int64_t)n_strokes * PG;
@@ -2071,10 +2103,9 @@ static int bwd_resident_blocks(int ns, int cid, const BwdShape& bs, int64_t* out
     return N3DS_OK;
 }
 
-// Backward blocks per scene, beyond what is resident at once, so the hardware
-// dispatcher evens out scenes and rays of uneven cost; each block gets at least
-// BATCH_MIN_RAYS rays per wavefront.
-constexpr int BATCH_OVERSUB = 8;
+// Backward blocks per scene: g_tune[TUNE_BATCH_OVERSUB] times what is resident at
+// once, so the hardware dispatcher evens out scenes and rays of uneven cost; each
+// block gets at least BATCH_MIN_RAYS rays per wavefront.
 constexpr int BATCH_MIN_RAYS = 4;
 
 N3DS_EXPORT int n3ds_render_batch_plan(int32_t* scenes, int n_scenes, int n_samples, int shape_width,
@@ -2118,6 +2149,17 @@ N3DS_EXPORT int n3ds_render_batch_plan(int32_t* scenes, int n_scenes, int n_samp
         st = bwd_resident_blocks(ns, color_id, bwd_shape(ns, color_id, sm, PG), &resident[c]);
         if (st) return -st;
     }
+    // culling tasks: each scene's rays x its own chunks, scene after scene
+    {
+        int64_t task = 0;
+        for (int i = 0; i < n_scenes; i++) {
+            int32_t* e = scenes + (int64_t)i * SCENE_REC;
+            N3DS_REQUIRE(task < (1ll << 31), "batch too large for the culling plan");
+            e[8] = (int32_t)task;
+            e[9] = 0;
+            task += (int64_t)e[1] * ((e[3] + 63) / 64);
+        }
+    }
     // blocks numbered class by class, scene by scene within a class; partial rows packed likewise
     int64_t b = 0, poff = 0;
     for (int c = 0; c < 4; c++) {
@@ -2128,7 +2170,7 @@ N3DS_EXPORT int n3ds_render_batch_plan(int32_t* scenes, int n_scenes, int n_samp
             if (e[1] > 0 && e[3] > 0) {
                 const double w = (double)e[1] * (e[3] + 32.0);
                 const int waves = e[7] ? e[7] : BWAVES;
-                g = (int64_t)((double)BATCH_OVERSUB * resident[c] * w / tot + 0.5);
+                g = (int64_t)((double)g_tune[TUNE_BATCH_OVERSUB] * resident[c] * w / tot + 0.5);
                 const int64_t cap = (e[1] + (int64_t)waves * BATCH_MIN_RAYS - 1) / ((int64_t)waves * BATCH_MIN_RAYS);
                 if (g > cap) g = cap;
                 if (g < 1) g = 1;
@@ -2173,7 +2215,14 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
-    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream);
+    int64_t ntask = 0;
+    for (int q = 0; q < n_scenes; q++) {
+        const int32_t* e = scenes_host + (int64_t)q * SCENE_REC;
+        N3DS_REQUIRE(e[8] == ntask, "scene %d has no valid culling plan (run n3ds_render_batch_plan)", q);
+        ntask += (int64_t)e[1] * ((e[3] + 63) / 64);
+    }
+    N3DS_REQUIRE(ntask < (1ll << 31), "too many (ray, chunk) culling tasks");
+    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream, ntask);
     if (st) return st;
     hipLaunchKernelGGL(A.ccounts ? fwd_pre_kernel_ptr(ns, color_id) : fwd_kernel_ptr(ns, color_id), grid, block,
                        fwd_lds_bytes(ns, color_id), (hipStream_t)stream,

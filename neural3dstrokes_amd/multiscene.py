@@ -26,13 +26,14 @@ from . import _lib
 from .optim import Adam
 from .render import render_strokes
 
-SCENE_REC = 8
+SCENE_REC = 10
 
 
 class SceneBatch:
     """Scene table of a packed batch (int32, host and device copies): per scene first
-    ray, rays, first stroke, strokes and the backward plan (first block, blocks,
-    partial-row offset, wavefronts per block), then the scene of every backward block."""
+    ray, rays, first stroke, strokes, the backward plan (first block, blocks,
+    partial-row offset, wavefronts per block) and the first culling task, then the
+    scene of every backward block."""
 
     def __init__(self, ray_counts, stroke_counts, n_samples, shape_width, color_width, color_id=0, device='cuda'):
         ray_counts = [int(r) for r in ray_counts]
@@ -52,6 +53,15 @@ class SceneBatch:
         self.n_rays = int(sum(ray_counts))
         self.n_strokes = int(sum(stroke_counts))
         self.s_max = int(max(stroke_counts))
+        self._rec = rec
+        self.device = device
+        self.replan()
+
+    def replan(self):
+        """(Re)computes the backward plan for the current device and tuning."""
+        rec, n = self._rec.copy(), self.n_scenes
+        n_samples, shape_width, color_width, color_id = (self.n_samples, self.shape_width, self.color_width,
+                                                         self.color_id)
         plan = _lib.lib().n3ds_render_batch_plan
         args = (n_samples, shape_width, color_width, color_id)
         nb = int(plan(rec.ctypes.data_as(ctypes.c_void_p), n, *args, None, 0))
@@ -66,7 +76,7 @@ class SceneBatch:
             _lib.check(-got if got < 0 else 1)
         self.n_blocks = nb
         self.records = self.table[:n * SCENE_REC].reshape(n, SCENE_REC)
-        self.device_table = torch.from_numpy(self.table.copy()).to(device)
+        self.device_table = torch.from_numpy(self.table.copy()).to(self.device)
 
     def host_ptr(self):
         return self.table.ctypes.data_as(ctypes.c_void_p)

@@ -158,3 +158,25 @@ def test_render_golden_fixture(case, dev):
     assert rel_err(sp.grad, t['grad_shape']) < 1e-4
     assert rel_err(cp.grad, t['grad_color']) < 1e-4
     assert rel_err(dp.grad, t['grad_density']) < 1e-4
+
+
+@pytest.mark.gpu
+def test_render_results_do_not_depend_on_tuning():
+    """Launch-shape parameters change only the float summation order of the gradients."""
+    from neural3dstrokes_amd import autotune
+    dev = torch.device('cuda:0')
+    sc = _scene('palette500_65536x128', 64, 128, 120, seed=9)
+    old = autotune.current()
+    try:
+        autotune.set('flush_pairs', 64)
+        autotune.set('bwd_oversub', 1)
+        (rgb0, acc0, d0), g0, _ = _run(sc, dev)
+        autotune.set('flush_pairs', 256)
+        autotune.set('bwd_oversub', 4)
+        (rgb1, acc1, d1), g1, _ = _run(sc, dev)
+    finally:
+        for k, v in old.items():
+            autotune.set(k, v)
+    assert torch.equal(rgb0, rgb1) and torch.equal(acc0, acc1) and torch.equal(d0, d1)
+    for a, b in zip(g0, g1):
+        assert rel_err(a, b) < 1e-5

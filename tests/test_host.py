@@ -40,7 +40,7 @@ def _header_symbols():
 
 def test_header_and_signature_table_agree():
     syms = _header_symbols()
-    assert len(syms) == 23
+    assert len(syms) == 24
     assert syms == sorted(_lib.SIGNATURES)
 
 
@@ -146,3 +146,19 @@ def test_flop_model_covers_the_palettes():
     for b in (0, 1, 2, 4, 7, 8, 9, 10):
         fwd, tot = bench.FLOPS[b]
         assert 0 < fwd < tot
+
+
+def test_render_tuning_parameters():
+    """n3ds_render_tuning reads / sets the launch-shape parameters and rejects bad ones (no device needed)."""
+    from neural3dstrokes_amd import autotune
+    old = autotune.current()
+    assert old['flush_pairs'] == 128 and old['bwd_oversub'] >= 1 and old['batch_oversub'] >= 1
+    assert autotune.set('flush_pairs', 192) == 128
+    assert autotune.get('flush_pairs') == 192
+    lib = _lib.lib()
+    assert lib.n3ds_render_tuning(0, 5000) == -1 and b'outside' in lib.n3ds_last_error()
+    assert lib.n3ds_render_tuning(7, 1) == -1
+    assert autotune.get('flush_pairs') == 192
+    for k, v in old.items():
+        autotune.set(k, v)
+    assert autotune.current() == old

@@ -21,7 +21,7 @@ def test_batch_plan_rejects_unpacked_tables():
     import ctypes
     from neural3dstrokes_amd import _lib
     lib = _lib.lib()
-    t = np.zeros((2, 8), dtype=np.int32)
+    t = np.zeros((2, 10), dtype=np.int32)
     t[0, 1], t[0, 3] = 10, 5
     t[1, 0], t[1, 1], t[1, 2], t[1, 3] = 11, 10, 5, 5  # second scene does not start after the first
     st = lib.n3ds_render_batch_plan(t.ctypes.data_as(ctypes.c_void_p), 2, 64, 14, 3, 0, None, 0)

@@ -1399,13 +1399,22 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
         const int64_t rp = ray * (int64_t)A.cap_p * (CID == 1 ? 4 : 1);
         int nh = 0, np = 0, gpoff = 0;
         bool gkept = false;
+        int d0 = -1, d1 = 0, d2 = 0;  // kept chunk records of chunks (ch & ~63) + lane
         for (int ch = 0; ch < A.nch;) {
             int hoff = -1, kpoff = 0, knc = 0;
             if (A.hdir) {
-                const int* e = A.hdir + (ray * A.nch_stride + ch) * 3;
-                hoff = __builtin_amdgcn_readfirstlane(e[0]);
-                kpoff = __builtin_amdgcn_readfirstlane(e[1]);
-                knc = __builtin_amdgcn_readfirstlane(e[2]);
+                if ((ch & 63) == 0 || ch == 0) {
+                    d0 = -1;
+                    if (ch + lane < A.nch) {
+                        const int* e = A.hdir + (ray * A.nch_stride + ch + lane) * 3;
+                        d0 = e[0];
+                        d1 = e[1];
+                        d2 = e[2];
+                    }
+                }
+                hoff = __builtin_amdgcn_readlane(d0, ch & 63);
+                kpoff = __builtin_amdgcn_readlane(d1, ch & 63);
+                knc = __builtin_amdgcn_readlane(d2, ch & 63);
             }
             const bool kept = hoff >= 0;
             bool do_flush = nh > 0 && kept != gkept;  // evaluate the pending hits before switching source

@@ -1002,9 +1002,53 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
     int nh = 0, np = 0;  // pending hits / pairs
     int gch = 0;         // first chunk of the pending group
     int cword = 0;       // culling pass: count word of chunk (ch & ~63) + lane
-    for (int ch = 0; ch < A.nch; ch++) {
-        int nc, tot;
+    int ch0 = 0;
+    bool loaded = false;  // all of the ray's hits are pending already
+    if constexpr (PRE) {
+        // when the ray's hits fit the pending list, gather them in one pass (one
+        // lane per hit; its chunk found by a binary search over the chunks' first hits)
+        if (A.nch <= 64) {
+            const int64_t slot0 = ray * A.nch_stride;
+            cword = lane < A.nch ? A.ccounts[slot0 + lane] : 0;
+            const int ncl = cword & 127;
+            const int hincl = wave_incl_scan(ncl, lane);
+            const int H = __shfl(hincl, 63, 64);
+            if (H <= HMAX) {
+                w.xo[lane] = hincl - ncl;
+                wave_sync();
+                int carry = 0;
+                for (int h0 = 0; h0 < H; h0 += 64) {
+                    const int hi = h0 + lane;
+                    int cnt = 0, c = 0, hw = 0;
+                    if (hi < H) {
+#pragma unroll
+                        for (int step = 32; step > 0; step >>= 1)
+                            if (c + step < A.nch && w.xo[c + step] <= hi) c += step;
+                        hw = A.cslots[(slot0 + c) * 64 + (hi - w.xo[c])];
+                        cnt = (hw >> 15) & 511;
+                    }
+                    const int incl = wave_incl_scan(cnt, lane);
+                    if (hi < H) {
+                        w.ck[hi] = A.S - 1 - (c * 64 + (hw & 63));
+                        w.ci0[hi] = (hw >> 6) & 511;
+                        w.ccnt[hi] = cnt;
+                        w.ccls[hi] = (hw >> 24) & 3;
+                        w.coff[hi] = carry + incl - cnt;
+                    }
+                    carry += __shfl(incl, 63, 64);
+                }
+                wave_sync();
+                nh = H;
+                np = carry;
+                ch0 = A.nch - 1;
+                loaded = true;
+            }
+        }
+    }
+    for (int ch = ch0; ch < A.nch; ch++) {
+        int nc = 0, tot = 0;
         if constexpr (PRE) {
+          if (!loaded) {
             // hits of the culling pass, final (capsules applied)
             const int64_t slot = ray * A.nch_stride + ch;
             if ((ch & 63) == 0) cword = ch + lane < A.nch ? A.ccounts[slot + lane] : 0;
@@ -1024,6 +1068,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
                 }
                 wave_sync();
             }
+          }
         } else {
             nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
         }

@@ -1445,10 +1445,50 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
         int nh = 0, np = 0, gpoff = 0;
         bool gkept = false;
         int d0 = -1, d1 = 0, d2 = 0;  // kept chunk records of chunks (ch & ~63) + lane
+        // every chunk kept and the hits fit the pending list: gather them in one pass (a
+        // ray's kept hits and pairs are contiguous from its first chunk's offsets)
+        bool preloaded = false;
+        if (A.hdir && A.nch <= 64) {
+            if (lane < A.nch) {
+                const int* e = A.hdir + (ray * A.nch_stride + lane) * 3;
+                d0 = e[0];
+                d1 = e[1];
+                d2 = e[2];
+            }
+            const bool all_kept = __ballot(lane < A.nch && d0 < 0) == 0ull;
+            int H = d2;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) H += __shfl_xor(H, off, 64);
+            if (all_kept && H <= HMAX) {
+                const int h00 = __builtin_amdgcn_readfirstlane(d0), p00 = __builtin_amdgcn_readfirstlane(d1);
+                int last = 0;
+                for (int i = lane; i < H; i += 64) {
+                    const int* h = A.hbuf + (rh + h00 + i) * 4;
+                    w.ck[i] = h[0] & 0x0fffffff;
+                    w.ccls[i] = (h[0] >> 28) & 3;
+                    w.ci0[i] = h[1];
+                    w.ccnt[i] = h[2];
+                    w.coff[i] = h[3] - p00;
+                    if (i == H - 1) last = h[3] - p00 + h[2];
+                }
+                wave_sync();
+                np = 0;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off, 64));
+                np = last;
+                nh = H;
+                gkept = true;
+                gpoff = p00;
+                preloaded = true;
+            }
+        }
         for (int ch = 0; ch < A.nch;) {
             int hoff = -1, kpoff = 0, knc = 0;
-            if (A.hdir) {
-                if ((ch & 63) == 0 || ch == 0) {
+            if (preloaded) {
+                preloaded = false;
+                ch = A.nch;  // the one flush below covers the whole ray
+            } else if (A.hdir) {
+                if ((ch & 63) == 0) {
                     d0 = -1;
                     if (ch + lane < A.nch) {
                         const int* e = A.hdir + (ray * A.nch_stride + ch + lane) * 3;
@@ -1462,7 +1502,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 knc = __builtin_amdgcn_readlane(d2, ch & 63);
             }
             const bool kept = hoff >= 0;
-            bool do_flush = nh > 0 && kept != gkept;  // evaluate the pending hits before switching source
+            bool do_flush = (nh > 0 && kept != gkept) || ch == A.nch;  // flush before switching source
             if (!do_flush) {
                 if (nh == 0) {
                     gkept = kept;

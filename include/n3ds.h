@@ -190,7 +190,9 @@ int n3ds_render_forward(const float* origins, const float* directions, const flo
  * (sample, stroke) pairs that trigger an evaluation window (16..256, default
  * 128), 1 = backward persistent blocks per resident block (1..8, default 1),
  * 2 = batch backward blocks per resident block (1..64, default 8; read by
- * n3ds_render_batch_plan).  value < 0 only queries.  Returns the previous
+ * n3ds_render_batch_plan), 3 = 64-stroke chunks per wavefront of the culling
+ * pass (1..16, default 1; a batch's plan depends on it).  value < 0 only
+ * queries.  Returns the previous
  * value, or -1 for an unknown parameter or a value out of range.  Results do
  * not depend on these parameters beyond float summation order. */
 int n3ds_render_tuning(int param, int value);

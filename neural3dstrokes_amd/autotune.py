@@ -6,7 +6,9 @@ Tunes, by timing the real workload on the device, the parameters exposed by
 * ``flush_pairs``: pending (sample, stroke) pairs that trigger an evaluation
   window (how many strokes' pairs are staged per window);
 * ``bwd_oversub``: backward persistent blocks per block resident at once;
-* ``batch_oversub``: a scene batch's backward blocks per resident block.
+* ``batch_oversub``: a scene batch's backward blocks per resident block;
+* ``cull_group``: 64-stroke chunks culled by one wavefront of the culling pass
+  (a scene batch's plan depends on it: replan after changing it).
 
 Results do not depend on them beyond float summation order.  ``tune`` runs a
 caller-supplied step over a small grid (coordinate descent, one parameter at a
@@ -17,8 +19,9 @@ import torch
 
 from . import _lib
 
-PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2}
-SPACE = {'flush_pairs': (64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16)}
+PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3}
+SPACE = {'flush_pairs': (64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16),
+         'cull_group': (1, 2, 4, 8)}
 _cache = {}
 
 

@@ -102,7 +102,8 @@ struct RenderArgs {
     // word (hits | pairs << 8) and up to 64 packed hits (pack_hit), index ray * nch_stride + chunk
     const int* ccounts;
     const int* cslots;
-    int64_t ntask;    // culling tasks: rays x chunks (a batch: each scene's own chunks)
+    int64_t ntask;    // culling tasks: rays x chunk groups (a batch: each scene's own chunks)
+    int cull_group;   // chunks culled by one culling task
     int flush_pairs;  // pending pairs that trigger evaluation
 };
 
@@ -930,23 +931,26 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t t = (int64_t)blockIdx.x * CWAVES + wave;
     if (t >= A0.ntask) return;
-    // ray-major: neighbouring wavefronts share the ray's depths and constants
+    // a task culls cull_group consecutive chunks of one ray; ray-major, so neighbouring
+    // wavefronts share the ray's depths and constants
     RenderArgs A = A0;
+    const int G = A0.cull_group;
     int64_t ray;
-    int ch;
-    if (A0.scenes) {  // a batch: tasks cover each scene's rays x its own chunks
+    int cg;
+    if (A0.scenes) {  // a batch: tasks cover each scene's rays x its own chunk groups
         const int* rec = A0.scenes + find_scene(A0.scenes, A0.n_scenes, 8, __builtin_amdgcn_readfirstlane((int)t)) *
                                          SCENE_REC;
         scene_strokes(A, rec);
         if (A.nch == 0) return;
+        const int ngr = (A.nch + G - 1) / G;
         const int local = (int)(t - rec[8]);
-        ray = rec[0] + local / A.nch;
-        ch = local - (local / A.nch) * A.nch;
+        ray = rec[0] + local / ngr;
+        cg = local - (local / ngr) * ngr;
     } else {
-        ray = t / A0.nch_stride;
-        ch = (int)(t - ray * A0.nch_stride);
+        const int ngr = (A0.nch_stride + G - 1) / G;
+        ray = t / ngr;
+        cg = (int)(t - ray * ngr);
     }
-    const int64_t slot = ray * A0.nch_stride + ch;
     WaveMem w;
     w.ts = smem + wave * cull_wave_floats(NS);
     w.pb = w.ts + 64 * NS;
@@ -954,13 +958,17 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     float ts[NS];
     bool valid[NS];
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
-    int k, i0, cnt, cls;
-    const bool hit = cull_chunk_final<NS>(A, rc, w, sorted, ch * 64, lane, k, i0, cnt, cls);
-    const unsigned long long mask = __ballot(hit);
-    const int incl = wave_incl_scan(hit ? cnt : 0, lane);
-    if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);
-    const int tot = __shfl(incl, 63, 64);
-    if (lane == 0) counts[slot] = __popcll(mask) | (tot << 8);
+    const int ch_end = min(A.nch, (cg + 1) * G);
+    for (int ch = cg * G; ch < ch_end; ch++) {
+        const int64_t slot = ray * A0.nch_stride + ch;
+        int k, i0, cnt, cls;
+        const bool hit = cull_chunk_final<NS>(A, rc, w, sorted, ch * 64, lane, k, i0, cnt, cls);
+        const unsigned long long mask = __ballot(hit);
+        const int incl = wave_incl_scan(hit ? cnt : 0, lane);
+        if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);
+        const int tot = __shfl(incl, 63, 64);
+        if (lane == 0) counts[slot] = __popcll(mask) | (tot << 8);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1844,11 +1852,11 @@ static int g_max_lds = 0;
 
 // Launch-shape parameters chosen by the autotuner (n3ds_render_tuning): pending pairs
 // that trigger an evaluation window, backward persistent blocks per resident block,
-// and the batch backward's blocks per resident block.
-enum { TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_COUNT = 3 };
-static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8};
-static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1};
-static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64};
+// the batch backward's blocks per resident block, and chunks per culling task.
+enum { TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_CULL_GROUP = 3, TUNE_COUNT = 4 };
+static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1};
+static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1};
+static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16};
 
 typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
 typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
@@ -1969,6 +1977,8 @@ static RenderArgs make_args(const float* origins, const float* dirs, const float
     A.ccounts = nullptr;
     A.cslots = nullptr;
     A.flush_pairs = g_tune[TUNE_FLUSH_PAIRS];
+    A.cull_group = g_tune[TUNE_CULL_GROUP];
+    A.ntask = 0;
     return A;
 }
 
@@ -2067,7 +2077,8 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
-    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream, n_rays * (int64_t)A.nch_stride);
+    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream,
+                     n_rays * (int64_t)((A.nch_stride + A.cull_group - 1) / A.cull_group));
     if (st) return st;
     hipLaunchKernelGGL(A.ccounts ? fwd_pre_kernel_ptr(ns, color_id) : fwd_kernel_ptr(ns, color_id), grid, block,
                        fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
@@ -2251,7 +2262,8 @@ N3DS_EXPORT int n3ds_render_batch_plan(int32_t* scenes, int n_scenes, int n_samp
             N3DS_REQUIRE(task < (1ll << 31), "batch too large for the culling plan");
             e[8] = (int32_t)task;
             e[9] = 0;
-            task += (int64_t)e[1] * ((e[3] + 63) / 64);
+            const int G = g_tune[TUNE_CULL_GROUP];
+            task += (int64_t)e[1] * (((e[3] + 63) / 64 + G - 1) / G);
         }
     }
     // blocks numbered class by class, scene by scene within a class; partial rows packed likewise
@@ -2313,7 +2325,7 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
     for (int q = 0; q < n_scenes; q++) {
         const int32_t* e = scenes_host + (int64_t)q * SCENE_REC;
         N3DS_REQUIRE(e[8] == ntask, "scene %d has no valid culling plan (run n3ds_render_batch_plan)", q);
-        ntask += (int64_t)e[1] * ((e[3] + 63) / 64);
+        ntask += (int64_t)e[1] * (((e[3] + 63) / 64 + A.cull_group - 1) / A.cull_group);
     }
     N3DS_REQUIRE(ntask < (1ll << 31), "too many (ray, chunk) culling tasks");
     st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream, ntask);

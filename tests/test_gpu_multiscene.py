@@ -131,3 +131,28 @@ def test_multiscene_step_gradients_match_per_scene_loop():
         gi = torch.autograd.grad(li, lv)
         for a, b in zip(gb, gi):
             assert _rel(a[ks], b) <= 1e-5
+
+
+def test_batch_plan_follows_cull_group():
+    """A batch planned with several chunks per culling task renders as the scenes alone."""
+    from neural3dstrokes_amd import autotune
+    from neural3dstrokes_amd.multiscene import SceneBatch, pack
+    from neural3dstrokes_amd.render import render_strokes
+    old = autotune.current()
+    try:
+        autotune.set('cull_group', 3)
+        sc, rc, kc = _batch_case(3, 17, 128, (60, 300), (64, 300), extra_empty=False)
+        packed = {k: v.to(DEV) for k, v in pack(sc).items()}
+        batch = SceneBatch(rc, kc, 128, 14, 3, 0, device=DEV)
+        out = render_strokes(*[packed[k] for k in KEYS], packed['stroke_ids'], packed['shape_params'],
+                             packed['color_params'], packed['density_params'], scenes=batch)
+        for i in range(batch.n_scenes):
+            rs, ks = batch.rays(i), batch.strokes(i)
+            o1 = render_strokes(*[packed[k][rs].contiguous() for k in KEYS], packed['stroke_ids'][ks].contiguous(),
+                                packed['shape_params'][ks], packed['color_params'][ks], packed['density_params'][ks])
+            for a, b in zip(out, o1):
+                assert torch.equal(a[rs], b)
+    finally:
+        for k, v in old.items():
+            autotune.set(k, v)
+

@@ -173,6 +173,7 @@ def test_render_results_do_not_depend_on_tuning():
         (rgb0, acc0, d0), g0, _ = _run(sc, dev)
         autotune.set('flush_pairs', 256)
         autotune.set('bwd_oversub', 4)
+        autotune.set('cull_group', 4)
         (rgb1, acc1, d1), g1, _ = _run(sc, dev)
     finally:
         for k, v in old.items():

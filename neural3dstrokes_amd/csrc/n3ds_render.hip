@@ -787,6 +787,40 @@ __device__ __forceinline__ int window_end(const WaveMem& w, int c0, int nh, int 
     return c0 + __popcll(__ballot(fits));
 }
 
+// The window's hits held one per lane (lane h: hit c0 + h) with their stroke's
+// density and uniform colour loaded in parallel, so the ordered loops read them
+// with v_readlane instead of a dependent LDS + scalar-memory round trip per hit.
+template <int CID>
+struct WindowHits {
+    int k, ci, cn, po;
+    float dk;
+    V3 cu;
+};
+
+template <int CID>
+__device__ __forceinline__ WindowHits<CID> window_hits(const RenderArgs& A, const WaveMem& w, int c0, int c1,
+                                                      int wbase, int lane, V3 vd) {
+    WindowHits<CID> h;
+    h.k = h.ci = h.cn = h.po = 0;
+    h.dk = 0.f;
+    h.cu = v3s(0.f);
+    const int c = c0 + lane;
+    if (c < c1) {
+        h.k = w.ck[c];
+        h.ci = w.ci0[c];
+        h.cn = w.ccnt[c];
+        h.po = w.coff[c] - wbase;
+        h.dk = A.dparams[h.k];
+        h.cu = stroke_color_uniform<CID>(A.cparams + (int64_t)h.k * A.Q, vd);
+    }
+    return h;
+}
+
+__device__ __forceinline__ int lane_i(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float lane_f(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
 // Number of pending hits (in compositing order) whose stroke lies in a chunk
 // before chunk cc; top = index of the first stroke in compositing order.
 __device__ __forceinline__ int first_hit_of_chunk(const WaveMem& w, int nh, int top, int cc) {
@@ -1161,13 +1195,15 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
             }
             wave_sync();
             // ordered over-compositing, strokes front to back
+            const WindowHits<CID> wh = window_hits<CID>(A, w, c0, c1, wbase, lane, rc.vd);
             for (int c = c0; c < c1; c++) {
-                const int k = __builtin_amdgcn_readfirstlane(w.ck[c]);
-                const int ci = __builtin_amdgcn_readfirstlane(w.ci0[c]);
-                const int cn = __builtin_amdgcn_readfirstlane(w.ccnt[c]);
-                const int po = __builtin_amdgcn_readfirstlane(w.coff[c] - wbase);
-                const float dk = A.dparams[k];
-                const V3 cu = stroke_color_uniform<CID>(A.cparams + (int64_t)k * A.Q, rc.vd);
+                const int j = c - c0;
+                const int k = lane_i(wh.k, j);
+                const int ci = lane_i(wh.ci, j);
+                const int cn = lane_i(wh.cn, j);
+                const int po = lane_i(wh.po, j);
+                const float dk = lane_f(wh.dk, j);
+                const V3 cu = v3(lane_f(wh.cu.x, j), lane_f(wh.cu.y, j), lane_f(wh.cu.z, j));
                 int tp = 0;
 #pragma unroll
                 for (int s = 0; s < NS; s++) {
@@ -1590,13 +1626,14 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 wave_sync();
                 // (2) compositing adjoint, strokes front to back; per pair it leaves
                 //     dL/dalpha, dL/d(density param) and dL/d(colour) in the staging buffer
+                const WindowHits<CID> wh = window_hits<CID>(A, w, c0, c1, wbase, lane, rc.vd);
                 for (int c = c0; c < c1; c++) {
-                    const int k = __builtin_amdgcn_readfirstlane(w.ck[c]);
-                    const int ci = __builtin_amdgcn_readfirstlane(w.ci0[c]);
-                    const int cnn = __builtin_amdgcn_readfirstlane(w.ccnt[c]);
-                    const int po = __builtin_amdgcn_readfirstlane(w.coff[c] - wbase);
-                    const float dk = A.dparams[k];
-                    const V3 cu = stroke_color_uniform<CID>(A.cparams + (int64_t)k * A.Q, rc.vd);
+                    const int j = c - c0;
+                    const int ci = lane_i(wh.ci, j);
+                    const int cnn = lane_i(wh.cn, j);
+                    const int po = lane_i(wh.po, j);
+                    const float dk = lane_f(wh.dk, j);
+                    const V3 cu = v3(lane_f(wh.cu.x, j), lane_f(wh.cu.y, j), lane_f(wh.cu.z, j));
 #pragma unroll
                     for (int s = 0; s < NS; s++) {
                         const int j = lane + 64 * s - ci;

@@ -950,8 +950,12 @@ __device__ __forceinline__ bool cull_chunk_final(const RenderArgs& A, const RayC
     return hit;
 }
 constexpr int CWAVES = 4;  // wavefronts per culling block
+constexpr int CBATCH = 4;  // chunks whose curve pieces are refined together
 
-__host__ __device__ constexpr int cull_wave_floats(int ns) { return 64 * ns + 256; }
+// per wave: depths [64 NS]; per (batch chunk, lane) interval lo / hi (order-preserving
+// ints) and state (hit | curve << 1 | passing pieces << 16); curve -> lane map [64];
+// capsule candidates [128]
+__host__ __device__ constexpr int cull_wave_floats(int ns) { return 64 * ns + CBATCH * 64 * 3 + 64 + 128; }
 
 // lane in chunk | first sample << 6 | samples << 15 | class << 24 (samples <= 256)
 __device__ __forceinline__ int pack_hit(int l, int i0, int cnt, int cls) {
@@ -987,21 +991,147 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     }
     WaveMem w;
     w.ts = smem + wave * cull_wave_floats(NS);
-    w.pb = w.ts + 64 * NS;
+    int* ilo = (int*)(w.ts + 64 * NS);  // [CBATCH][64]
+    int* ihi = ilo + CBATCH * 64;
+    int* ist = ihi + CBATCH * 64;
+    int* src = ist + CBATCH * 64;       // [64]
+    int* cand = src + 64;               // [128]
     const RayConst rc = load_ray(A, ray);
     float ts[NS];
     bool valid[NS];
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
     const int ch_end = min(A.nch, (cg + 1) * G);
-    for (int ch = cg * G; ch < ch_end; ch++) {
-        const int64_t slot = ray * A0.nch_stride + ch;
-        int k, i0, cnt, cls;
-        const bool hit = cull_chunk_final<NS>(A, rc, w, sorted, ch * 64, lane, k, i0, cnt, cls);
-        const unsigned long long mask = __ballot(hit);
-        const int incl = wave_incl_scan(hit ? cnt : 0, lane);
-        if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);
-        const int tot = __shfl(incl, 63, 64);
-        if (lane == 0) counts[slot] = __popcll(mask) | (tot << 8);
+    for (int cb = cg * G; cb < ch_end; cb += CBATCH) {
+        const int nb = min(CBATCH, ch_end - cb);
+        // (1) per chunk: cull every stroke; for hull-hit curves test each piece's chord box,
+        //     one lane per piece, and keep the mask of passing pieces
+        for (int j = 0; j < nb; j++) {
+            const int base0 = (cb + j) * 64;
+            const int k = A.S - 1 - (base0 + lane);
+            float ta = 0.f, tb = -1.f;
+            bool curve = false;
+            const bool hit = k >= 0 && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr,
+                                                   rc.tlo, rc.thi, ta, tb, curve);
+            const unsigned long long cm = __ballot(hit && curve);
+            int pmask = 0;
+            if (cm) {
+                const int crank = __popcll(cm & ((1ull << lane) - 1ull));
+                if (hit && curve) src[crank] = lane;
+                wave_sync();
+                const int ncv = __popcll(cm);
+                const int kb = A.S - 1 - base0;
+                for (int q0 = 0; q0 < ncv; q0 += 64 / NPIECE) {
+                    const int cs = q0 + lane / NPIECE;
+                    const int piece = lane % NPIECE;
+                    bool bh = false;
+                    if (cs < ncv) {
+                        const float* rec = A.table + (int64_t)(kb - src[cs]) * REC;
+                        const float e = rec[R_RMAX] + rc.m + rec[R_DEV + piece];
+                        const V3 a = ld3(rec + R_PTS + 3 * piece), b = ld3(rec + R_PTS + 3 * piece + 3);
+                        float p0 = rc.tlo, p1 = rc.thi;
+                        slab_inv(rc.oc.x, rc.di.x, fminf(a.x, b.x) - e, fmaxf(a.x, b.x) + e, p0, p1);
+                        slab_inv(rc.oc.y, rc.di.y, fminf(a.y, b.y) - e, fmaxf(a.y, b.y) + e, p0, p1);
+                        slab_inv(rc.oc.z, rc.di.z, fminf(a.z, b.z) - e, fmaxf(a.z, b.z) + e, p0, p1);
+                        bh = p0 <= p1;
+                    }
+                    const unsigned long long bm = __ballot(bh);
+                    if (cs < ncv && piece == 0) cand[cs] = (int)((bm >> (lane & ~(NPIECE - 1))) & 0xffffull);
+                }
+                wave_sync();
+                if (hit && curve) pmask = cand[crank];
+                wave_sync();
+            }
+            const bool live = hit && (!curve || pmask != 0);
+            ilo[j * 64 + lane] = curve ? ord_int(1e30f) : ord_int(ta);
+            ihi[j * 64 + lane] = curve ? ord_int(-1e30f) : ord_int(tb);
+            ist[j * 64 + lane] = (live ? 1 : 0) | (curve ? 2 : 0) | (pmask << 16);
+        }
+        wave_sync();
+        // (2) the passing pieces of all chunks of the batch against their chord capsules, 64
+        //     at a time; each hit capsule widens its curve's interval (LDS min / max)
+        int nc = 0;  // pending candidates, < 64 between rounds
+        for (int j = 0; j <= nb; j++) {
+            unsigned pm = 0u;
+            int np = 0, incl = 0, tot = 0;
+            if (j < nb) {
+                const int st = ist[j * 64 + lane];
+                pm = (st & 1) ? ((unsigned)st >> 16) : 0u;
+                np = __popc(pm);
+                incl = wave_incl_scan(np, lane);
+                tot = __shfl(incl, 63, 64);
+            }
+            int done = 0;
+            do {
+                // append up to the free buffer space of this chunk's candidates
+                const int take = min(tot - done, 128 - nc);
+                int idx = incl - np;
+                for (unsigned m = pm; m; m &= m - 1, idx++)
+                    if (idx >= done && idx < done + take)
+                        cand[nc + idx - done] = (j << 10) | (lane << 4) | __builtin_ctz(m);
+                nc += take;
+                done += take;
+                wave_sync();
+                // capsule rounds: full ones while filling, the rest after the last chunk
+                while (nc >= 64 || (j == nb && nc > 0)) {
+                    const int nr = min(nc, 64);
+                    if (lane < nr) {
+                        const int cd = cand[lane];
+                        const int cj = cd >> 10, cl = (cd >> 4) & 63, pc = cd & 15;
+                        const float* rec = A.table + (int64_t)(A.S - 1 - ((cb + cj) * 64 + cl)) * REC;
+                        float p0, p1;
+                        if (capsule_interval(rc.oc, rc.dc, rc.idd, ld3(rec + R_PTS + 3 * pc),
+                                             ld3(rec + R_PTS + 3 * pc + 3), rec[R_RMAX] + rc.m + rec[R_DEV + pc], p0,
+                                             p1)) {
+                            p0 = fmaxf(p0, rc.tlo);
+                            p1 = fminf(p1, rc.thi);
+                            if (p0 <= p1) {
+                                atomicMin(&ilo[cj * 64 + cl], ord_int(p0));
+                                atomicMax(&ihi[cj * 64 + cl], ord_int(p1));
+                            }
+                        }
+                    }
+                    const int rest = nc - nr;
+                    const int mv = lane < rest ? cand[64 + lane] : 0;
+                    wave_sync();
+                    if (lane < rest) cand[lane] = mv;
+                    nc = rest;
+                    wave_sync();
+                }
+            } while (done < tot);
+        }
+        wave_sync();
+        // (3) per chunk: the sample ranges, then the compacted hits
+        for (int j = 0; j < nb; j++) {
+            const int ch = cb + j;
+            const int64_t slot = ray * A0.nch_stride + ch;
+            const int k = A.S - 1 - (ch * 64 + lane);
+            const int st = ist[j * 64 + lane];
+            bool hit = (st & 1) != 0;
+            int i0 = 0, cnt = 0, cls = 2;
+            if (hit) {
+                float ta = ord_float(ilo[j * 64 + lane]), tb = ord_float(ihi[j * 64 + lane]);
+                if (st & 2) {  // curve: the hull of its hit capsules
+                    hit = ta <= tb;
+                    if (hit) pad_interval(ta, tb, ta, tb);
+                }
+                if (hit) {
+                    if (sorted) {
+                        i0 = count_below<NS, false>(w.ts, ta);
+                        cnt = count_below<NS, true>(w.ts, tb) - i0;
+                    } else {
+                        cnt = A.n;
+                    }
+                    hit = cnt > 0;
+                    cls = stroke_class(A.table + (int64_t)k * REC, (st & 2) != 0);
+                }
+            }
+            const unsigned long long mask = __ballot(hit);
+            const int incl = wave_incl_scan(hit ? cnt : 0, lane);
+            if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);
+            const int tot = __shfl(incl, 63, 64);
+            if (lane == 0) counts[slot] = __popcll(mask) | (tot << 8);
+        }
+        wave_sync();
     }
 }
 

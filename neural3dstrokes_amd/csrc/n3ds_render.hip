@@ -641,6 +641,29 @@ __device__ __forceinline__ bool stage_depths(const RenderArgs& A, int64_t ray, i
     return __ballot(bad) == 0ull;
 }
 
+// Unsorted depths: the segment the strokes are culled against spans the smallest and
+// largest sample depth (not the first and last), and the cone margin follows it.
+template <int NS>
+__device__ __forceinline__ void unsorted_range(const RenderArgs& A, RayConst& rc, const float (&ts)[NS],
+                                               const bool (&valid)[NS]) {
+    float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (valid[s]) {
+            lo = fminf(lo, ts[s]);
+            hi = fmaxf(hi, ts[s]);
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+    }
+    rc.tlo = lo;
+    rc.thi = hi;
+    const float rmax = A.cone ? rc.radii * fmaxf(fabsf(lo), fabsf(hi)) : rc.radii;
+    rc.m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
+}
+
 // Culls the 64 strokes of the chunk starting at base0 (in compositing order),
 // one per lane: stroke k, whether it is hit, its samples [i0, i0 + cnt), its
 // class and, for a curve, the mask of its pieces whose chord boxes pass (the
@@ -996,10 +1019,11 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
     int* ist = ihi + CBATCH * 64;
     int* src = ist + CBATCH * 64;       // [64]
     int* cand = src + 64;               // [128]
-    const RayConst rc = load_ray(A, ray);
+    RayConst rc = load_ray(A, ray);
     float ts[NS];
     bool valid[NS];
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
+    if (!sorted) unsorted_range<NS>(A, rc, ts, valid);
     const int ch_end = min(A.nch, (cg + 1) * G);
     for (int cb = cg * G; cb < ch_end; cb += CBATCH) {
         const int nb = min(CBATCH, ch_end - cb);
@@ -1156,10 +1180,11 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
         scene_strokes(A, A0.scenes + sc * SCENE_REC);
     }
     const WaveMem w = wave_mem(smem + wave * wave_mem_floats(NS, NPF, 0), NS, NPF);
-    const RayConst rc = load_ray(A, ray);
+    RayConst rc = load_ray(A, ray);
     float ts[NS];
     bool valid[NS];
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
+    if (!sorted) unsorted_range<NS>(A, rc, ts, valid);
     float T[NS], Dn[NS];
     V3 C[NS];
 #pragma unroll
@@ -1519,10 +1544,11 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     const WaveMem w = wave_mem(smem + nacc_max + wave * wave_mem_floats(NS, NPF, NG), NS, NPF, NG);
     const int64_t N = A.R * (int64_t)A.n;
     for (int64_t ray = ray0; ray < ray_end; ray += ray_step) {
-        const RayConst rc = load_ray(A, ray);
+        RayConst rc = load_ray(A, ray);
         float ts[NS];
         bool valid[NS];
         const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
+        if (!sorted) unsorted_range<NS>(A, rc, ts, valid);
         const float* td = A.tdist + ray * (A.n + 1);
         const float dn = sqrtf(rc.d.x * rc.d.x + rc.d.y * rc.d.y + rc.d.z * rc.d.z);
         // ---- ray-march adjoint from the saved per-sample state

@@ -181,3 +181,30 @@ def test_render_results_do_not_depend_on_tuning():
     assert torch.equal(rgb0, rgb1) and torch.equal(acc0, acc1) and torch.equal(d0, d1)
     for a, b in zip(g0, g1):
         assert rel_err(a, b) < 1e-5
+
+
+@pytest.mark.gpu
+def test_render_many_chunks_culling_pass_vs_inline(dev):
+    """More than 64 chunks of 64 strokes (4300 strokes): the culling pass + pre-culled
+    forward + kept adjoint agree with inline culling + recomputation."""
+    sc = _scene('palette500_65536x128', 24, 64, 4300, seed=13)
+    (rgb0, acc0, d0), g0, _ = _run(sc, dev, store='auto')
+    (rgb1, acc1, d1), g1, _ = _run(sc, dev, store=False)
+    assert float(acc0.detach().max()) > 0.1  # the rays do cross strokes
+    assert torch.equal(rgb0, rgb1) and torch.equal(acc0, acc1) and torch.equal(d0, d1)
+    for a, b in zip(g0, g1):
+        assert rel_err(a, b) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('store', ['auto', False])
+def test_render_unsorted_depths(store, dev):
+    """Fenceposts given in decreasing order (no binary search over the samples: every
+    sample of a hit stroke is evaluated) still match the oracle."""
+    sc = _scene('palette500_65536x128', 16, 64, 40, seed=21)
+    sc['tdist'] = sc['tdist'].flip(-1).contiguous()
+    (rgb, acc, _), (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev, store=store)
+    out, rs, rc, rd = ofield.render_field_grads(g_rgb.to(D64), g_acc.to(D64), *_oracle_args(sc))
+    assert rel_err(rgb, out['rgb']) < 1e-5
+    assert rel_err(acc, out['acc']) < 1e-5
+    assert rel_err(gs, rs) < 1e-4 and rel_err(gc, rc) < 1e-4 and rel_err(gd, rd) < 1e-4

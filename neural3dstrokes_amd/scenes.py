@@ -192,7 +192,7 @@ def make_workload(name, seed=0, n_rays=None, n_samples=None, n_strokes=None):
 
 
 def make_multiscene(n_scenes=32, seed=0, stroke_range=(20, 1000), ray_range=(1024, 65536), n_samples=128,
-                    shape_width=14):
+                    shape_width=14, color_id=0):
     """Independent scenes with log-uniform stroke and ray counts, each drawn like the
     500-stroke mixed palette (40% cubic, 20% quadratic Bezier, 40% solids) with its own
     cameras; shape params zero-padded to ``shape_width`` so the scenes pack together.
@@ -205,7 +205,7 @@ def make_multiscene(n_scenes=32, seed=0, stroke_range=(20, 1000), ray_range=(102
     scenes = []
     for S, R in zip(s_counts, r_counts):
         types = palette(S, gen, 'mixed')
-        sc = make_strokes(types, gen)
+        sc = make_strokes(types, gen, color_id=color_id)
         sp = sc['shape_params']
         if sp.shape[1] < shape_width:
             sc['shape_params'] = torch.cat([sp, sp.new_zeros(S, shape_width - sp.shape[1])], 1)

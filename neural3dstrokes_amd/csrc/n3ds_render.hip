@@ -56,6 +56,8 @@ enum : int {
     R_E0 = 29,      // 3: unit box half extent
     R_E1 = 32,      // 3: margin multiplier
     R_M = 36,       // 9: culling map diag(1/s) R^T (row-major)
+    R_SZ = 45,      // sin / cos of the z angle (Euler adjoint)
+    R_CZ = 46,
     R_CP = 8,       // 14: curve params (curves reuse the solid slots)
     R_PTS = 22,     // (NPIECE + 1) * 3: curve points at the piece ends
     R_DEV = 73,     // NPIECE: bound on the distance of a piece from its chord
@@ -200,6 +202,7 @@ __global__ void render_prep_kernel(float* __restrict__ table, const int* __restr
         rec[R_TR + 0] = F.trans.x; rec[R_TR + 1] = F.trans.y; rec[R_TR + 2] = F.trans.z;
         rec[R_SC + 0] = F.scale.x; rec[R_SC + 1] = F.scale.y; rec[R_SC + 2] = F.scale.z;
         rec[R_ANG + 0] = F.angle.x; rec[R_ANG + 1] = F.angle.y; rec[R_ANG + 2] = F.angle.z;
+        rec[R_SZ] = F.sz; rec[R_CZ] = F.cz;
         for (int i = 0; i < nb; i++) rec[R_BP + i] = row[i];
         rec[R_MINS] = fminf(F.scale.x, fminf(F.scale.y, F.scale.z));
         const float isc[3] = {1.f / F.scale.x, 1.f / F.scale.y, 1.f / F.scale.z};
@@ -1921,6 +1924,8 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                                 Frame F;
                                 F.trans = ld3(rec + R_TR);
                                 F.angle = ld3(rec + R_ANG);
+                                F.sz = rec[R_SZ];
+                                F.cz = rec[R_CZ];
                                 F.scale = ld3(rec + R_SC);
 #pragma unroll
                                 for (int ii = 0; ii < 3; ii++)

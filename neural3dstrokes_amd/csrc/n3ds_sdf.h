@@ -70,10 +70,10 @@ __device__ __forceinline__ V3 rot_n(const Rot& R, V3 p) {
 // dL/d(euler) for p = Rᵀ q given g = dL/dp.  With w = (R g) × q:
 // dL/dx = R[:,0]·w, dL/dy = (-sin z, cos z, 0)·w, dL/dz = w.z.  (Closed form of
 // sum_ij g_j q_i dR_ij/da for R = Rz Ry Rx.)
-__device__ __forceinline__ V3 grad_euler(const Rot& R, V3 angle, V3 g, V3 q) {
+// sz, cz: sin / cos of the z angle, computed once per stroke.
+__device__ __forceinline__ V3 grad_euler(const Rot& R, float sz, float cz, V3 g, V3 q) {
     V3 Rg = rot_n(R, g);
     V3 w = cross(Rg, q);
-    float sz = sinf(angle.z), cz = cosf(angle.z);
     return v3(R.r[0][0] * w.x + R.r[1][0] * w.y + R.r[2][0] * w.z, -sz * w.x + cz * w.y, w.z);
 }
 

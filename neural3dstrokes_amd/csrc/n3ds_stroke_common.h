@@ -21,6 +21,7 @@ struct Frame {
     V3 angle;
     Rot R;
     V3 scale;  // SS: replicated
+    float sz, cz;  // sin / cos of the z angle (the Euler adjoint's second axis)
     int off_scale, off_angle, off_trans;
 };
 
@@ -31,9 +32,18 @@ __device__ __forceinline__ Frame load_frame(const float* row, int P, uint32_t fl
     F.trans = v3s(0.f);
     F.angle = v3s(0.f);
     F.scale = v3s(1.f);
+    F.sz = 0.f;
+    F.cz = 1.f;
     F.off_trans = F.off_angle = F.off_scale = -1;
     if (flags & F_T) { end -= 3; F.trans = ld3(row + end); F.off_trans = end; }
-    if (flags & F_R) { end -= 3; F.angle = ld3(row + end); F.off_angle = end; F.R = make_rot(F.angle); }
+    if (flags & F_R) {
+        end -= 3;
+        F.angle = ld3(row + end);
+        F.off_angle = end;
+        F.R = make_rot(F.angle);
+        F.sz = sinf(F.angle.z);
+        F.cz = cosf(F.angle.z);
+    }
     if (flags & F_SS) { end -= 1; float s = row[end]; F.scale = v3(s, s, s); F.off_scale = end; }
     else if (flags & F_MS) { end -= 3; F.scale = ld3(row + end); F.off_scale = end; }
     return F;
@@ -86,7 +96,7 @@ __device__ __forceinline__ V3 xform_adjoint(V3 gp, V3 p, V3 x, const Frame& F, u
     }
     if (flags & F_R) {
         V3 q = (flags & F_T) ? x - F.trans : x;
-        gang = gang + grad_euler(F.R, F.angle, g, q);
+        gang = gang + grad_euler(F.R, F.sz, F.cz, g, q);
         g = rot_n(F.R, g);
     }
     if (flags & F_T) gtr = gtr - g;

@@ -239,6 +239,80 @@ def grid_block(seed=0, n_points=1 << 20, steps=20):
             'peak_GBps': 8000.0, **res}
 
 
+def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
+    """Image texture on stroke colours (256x256 RGB, tint_ratio 0.7, alpha modulated) over the
+    (sample, stroke) pairs of 2048 rays x 128 samples x 500 strokes: fwd / bwd kernel times,
+    GB/s against 8 TB/s, the same ops in torch float32 for comparison, and errors against the
+    float64 oracle on the first 65536 pairs."""
+    import numpy as np
+    import torch
+    import torch.nn.functional as F
+    from neural3dstrokes_amd import _lib
+    from neural3dstrokes_amd.textures import ImageTexture
+    from oracle import texture as otex
+    dev = torch.device('cuda:0')
+    g = torch.Generator().manual_seed(seed)
+    img = (torch.rand(256, 256, 3, generator=g) * 255).round().numpy()
+    tex = ImageTexture(img, (256, 256), tint_ratio=0.7, modulate_alpha=True).to(dev)
+    tc = torch.rand(n_pairs, 2, device=dev)
+    col = torch.rand(n_pairs, 3, device=dev)
+    al = torch.rand(n_pairs, device=dev)
+    gc = torch.randn(n_pairs, 3, device=dev)
+    ga = torch.randn(n_pairs, device=dev)
+    col_o, al_o = torch.empty_like(col), torch.empty_like(al)
+    g_col, g_al, g_tc = torch.empty_like(col), torch.empty_like(al), torch.empty_like(tc)
+    lib = _lib.lib()
+    T = tex.texels
+
+    def fwd():
+        _lib.check(lib.n3ds_texture_forward(_lib.ptr(col_o), _lib.ptr(al_o), _lib.ptr(tc), _lib.ptr(col), _lib.ptr(al),
+                                            _lib.ptr(T), 256, 256, 3, n_pairs, 0.7, 1, _lib.stream()))
+
+    def bwd():
+        _lib.check(lib.n3ds_texture_backward(_lib.ptr(g_col), _lib.ptr(g_al), _lib.ptr(g_tc), _lib.ptr(gc),
+                                             _lib.ptr(ga), _lib.ptr(tc), _lib.ptr(col), _lib.ptr(al), _lib.ptr(T), 256,
+                                             256, 3, n_pairs, 0.7, 1, _lib.stream()))
+
+    def torch_ops():
+        t, c, a = (x.detach().requires_grad_(True) for x in (tc, col, al))
+        tint = F.grid_sample(tex.image_texture, t.reshape(1, 1, -1, 2) * 2 - 1, align_corners=True)[0, :, 0, :].T
+        cc = (c * 0.7 + 0.3) * tint
+        aa = a * tint.mean(-1)
+        torch.autograd.backward([cc, aa], [gc, ga])
+
+    times = {}
+    for name, fn, reps in (('fwd', fwd, steps), ('bwd', bwd, steps), ('torch_fwd_bwd', torch_ops, 2)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / reps
+    b_fwd = n_pairs * 40  # texcoords 8 + colour 12 + alpha 4 in, colour 12 + alpha 4 out
+    b_bwd = n_pairs * 64  # texcoords, colour, alpha, upstream colour / alpha in; three gradients out
+    k = 65536
+    d = [x[:k].double().cpu().numpy() for x in (tc, col, al)]
+    image = tex.image_texture[0].double().cpu().numpy()
+    oc, oa = otex.texture_forward(image, *d, 0.7, True)
+    og = otex.texture_backward(image, *d, gc[:k].double().cpu().numpy(), ga[:k].double().cpu().numpy(), 0.7, True)
+
+    def r(a, b):
+        return float(np.linalg.norm(a[:k].double().cpu().numpy() - b) / np.linalg.norm(b))
+
+    return {'workload': f'{n_pairs} (sample, stroke) pairs = 2048 rays x 128 samples x 500 strokes, 256x256 RGB '
+                        'texture, tint_ratio 0.7, alpha modulated, texcoord gradient on',
+            'fwd_ms': times['fwd'], 'bwd_ms': times['bwd'], 'torch_ops_fwd_bwd_ms': times['torch_fwd_bwd'],
+            'speedup_vs_torch_ops': times['torch_fwd_bwd'] / (times['fwd'] + times['bwd']),
+            'fwd_bytes': b_fwd, 'bwd_bytes': b_bwd,
+            'fwd_GBps': b_fwd / (times['fwd'] * 1e-3) / 1e9, 'bwd_GBps': b_bwd / (times['bwd'] * 1e-3) / 1e9,
+            'fwd_frac_hbm': b_fwd / (times['fwd'] * 1e-3) / 8e12, 'bwd_frac_hbm': b_bwd / (times['bwd'] * 1e-3) / 8e12,
+            'rel_err_vs_fp64': {'colors': r(col_o, oc), 'alphas': r(al_o, oa), 'grad_colors': r(g_col, og[0]),
+                                'grad_alphas': r(g_al, og[1]), 'grad_texcoords': r(g_tc, og[2])}}
+
+
 def memory_block(steps=3, warmup=1):
     """640,000 rays x 192 samples x 1000 strokes: fused fwd+bwd with the forward's hits / alpha
     kept for the adjoint versus full recomputation; throughput and peak HBM for both."""
@@ -425,6 +499,7 @@ def main():
     ap.add_argument('--acc-rays', type=int, default=4096, help='rays for the float64 accuracy check (0: skip)')
     ap.add_argument('--fd-params', type=int, default=32, help='parameters for the finite-difference check (0: skip)')
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
+    ap.add_argument('--texture', type=int, default=1, help='image-texture block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
     ap.add_argument('--autotune', type=int, default=1, help='tune launch shapes on the workload first (0: defaults)')
@@ -572,6 +647,8 @@ def main():
         out['finite_differences'] = fd_block(sc, T, ids, gr, ga, n_params=args.fd_params)
     if args.grid:
         out['hash_grid'] = grid_block()
+    if args.texture:
+        out['image_texture'] = texture_block()
     if args.memory:
         out['memory_bounded'] = memory_block()
     if args.multiscene:

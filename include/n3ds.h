@@ -314,6 +314,37 @@ int n3ds_adam_step(float* params, float* grads, float* exp_avg, float* exp_avg_s
                    double beta2, double eps, int64_t step, const float* step_dev, float grad_max_val,
                    const float* grad_coef, float lo_all, float hi_all, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Image texture on stroke colours.
+ *
+ * Replaces the torch ops of ImageTexture.forward / sample_texture
+ * (source/textures.py:51-82; applied between the stroke and compositing
+ * operators at models.py:704-706):
+ *   tint = grid_sample(image, texcoords * 2 - 1, bilinear, zeros padding,
+ *                      align_corners=True);
+ *   colors_out = (colors * tint_ratio + (1 - tint_ratio)) * tint;
+ *   alphas_out = alphas * mean(tint)   (only when modulate_alpha).
+ * texture [H, W, 4] float (channel c of texel (y, x) at (y*W + x)*4 + c,
+ * 16-byte aligned; the first `channels` in {1, 3} are used; H*W <= 2^26),
+ * texcoords [N, 2] (u along W, v along H), colors / colors_out [N, 3],
+ * alphas / alphas_out [N] (read / written only when modulate_alpha).
+ * ------------------------------------------------------------------------ */
+int n3ds_texture_forward(float* colors_out, float* alphas_out, const float* texcoords, const float* colors,
+                         const float* alphas, const float* texture, int64_t height, int64_t width, int64_t channels,
+                         int64_t n, double tint_ratio, int modulate_alpha, void* stream);
+
+/* Adjoint of n3ds_texture_forward: what torch's autograd gives the ops of
+ * textures.py:51-82 (the image is a non-persistent buffer without gradient).
+ * grad_colors [N,3] and, when modulate_alpha, grad_alphas [N] are written;
+ * grad_texcoords [N,2] is written when not NULL (the reference's stroke
+ * backward discards it, strokes.py:188-224).  grad_colors_out /
+ * grad_alphas_out may be NULL (zeros). */
+int n3ds_texture_backward(float* grad_colors, float* grad_alphas, float* grad_texcoords,
+                          const float* grad_colors_out, const float* grad_alphas_out, const float* texcoords,
+                          const float* colors, const float* alphas, const float* texture, int64_t height,
+                          int64_t width, int64_t channels, int64_t n, double tint_ratio, int modulate_alpha,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

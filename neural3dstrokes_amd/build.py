@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libn3ds.so')
 BUILD_DIR = os.path.join(HERE, '_build')
-SOURCES = ['n3ds_capi.cpp', 'n3ds_strokes.hip', 'n3ds_compose.hip', 'n3ds_grid.hip', 'n3ds_render.hip', 'n3ds_optim.hip']
+SOURCES = ['n3ds_capi.cpp', 'n3ds_strokes.hip', 'n3ds_compose.hip', 'n3ds_grid.hip', 'n3ds_render.hip', 'n3ds_optim.hip',
+           'n3ds_texture.hip']
 ARCH = os.environ.get('N3DS_OFFLOAD_ARCH', 'gfx950')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 

@@ -1,0 +1,214 @@
+// Image texture on stroke colours: drop-in replacement for the torch ops of the
+// reference's ImageTexture (source/textures.py:36-82):
+//   tint   = grid_sample(image, texcoords * 2 - 1, bilinear, zero padding,
+//                        align_corners=True)                      (:51-61)
+//   colour = (colour * tint_ratio + (1 - tint_ratio)) * tint        (:77-78)
+//   alpha  = alpha * mean(tint)          when modulate_alpha        (:79-80)
+//
+// Layout: the texture sits in HBM as [H][W] float4 (channels 0..C-1 used,
+// C = 1 or 3), so one corner is one 16-byte gather; a 256x256 texture is
+// 1 MiB and stays resident in each XCD's L2.  One thread per (sample, stroke)
+// pair in a grid-stride loop; per pair the kernel moves 8 B of texcoords,
+// 12 B of colour in and out, and 4 B of alpha in and out when modulating, so
+// it is HBM bound at large N.
+//
+// Derivatives are those torch's autograd gives the same ops (the image is a
+// buffer without gradient): d colour_in = g * tint * tint_ratio,
+// d alpha_in = g_alpha * mean(tint), and d texcoords from the bilinear
+// weights with out-of-range corners contributing zero on both sides.
+#include <algorithm>
+
+#include "n3ds_capi.h"
+#include "n3ds_math.h"
+
+namespace n3ds {
+
+constexpr int TEX_THREADS = 256;
+
+struct TexSample {
+    float t[3];        // tint per channel (channel 0 only when C == 1)
+    float dx[3], dy[3];  // derivatives w.r.t. the unnormalised pixel coordinates ix, iy
+};
+
+// Bilinear tap of an [H][W] float4 texture at texcoords (u along W, v along H)
+// in [0, 1]; corners outside the texture contribute zero.
+template <int C, bool GRAD>
+__device__ __forceinline__ TexSample tex_sample(const float4* __restrict__ tex, int H, int W, float u, float v) {
+    // the reference maps [0, 1] to [-1, 1] (uvs = texcoords * 2 - 1) and
+    // grid_sample maps that back to pixels ((g + 1) / 2) * (size - 1)
+    const float gu = __fsub_rn(__fmul_rn(u, 2.f), 1.f), gv = __fsub_rn(__fmul_rn(v, 2.f), 1.f);
+    const float ix = __fmul_rn(__fadd_rn(gu, 1.f) * 0.5f, (float)(W - 1));
+    const float iy = __fmul_rn(__fadd_rn(gv, 1.f) * 0.5f, (float)(H - 1));
+    // a floor outside [-2, size] leaves every corner outside (NaN lands there too),
+    // and keeps the integer corner indices far from overflow
+    const float fx = fminf(fmaxf(floorf(ix), -2.f), (float)W), fy = fminf(fmaxf(floorf(iy), -2.f), (float)H);
+    const int x0 = (int)fx, y0 = (int)fy;
+    const float ex = __fsub_rn(fx + 1.f, ix), ox = __fsub_rn(ix, fx);  // x1 - ix, ix - x0
+    const float ey = __fsub_rn(fy + 1.f, iy), oy = __fsub_rn(iy, fy);
+    const bool inx0 = x0 >= 0 && x0 < W, inx1 = x0 + 1 >= 0 && x0 + 1 < W;
+    const bool iny0 = y0 >= 0 && y0 < H, iny1 = y0 + 1 >= 0 && y0 + 1 < H;
+    // corner order nw (x0,y0), ne (x1,y0), sw (x0,y1), se (x1,y1)
+    const bool in[4] = {inx0 && iny0, inx1 && iny0, inx0 && iny1, inx1 && iny1};
+    const int cx[4] = {x0, x0 + 1, x0, x0 + 1}, cy[4] = {y0, y0, y0 + 1, y0 + 1};
+    const float w[4] = {__fmul_rn(ex, ey), __fmul_rn(ox, ey), __fmul_rn(ex, oy), __fmul_rn(ox, oy)};
+    const float wdx[4] = {-ey, ey, -oy, oy}, wdy[4] = {-ex, -ox, ex, ox};
+    TexSample s;
+#pragma unroll
+    for (int c = 0; c < 3; c++) s.t[c] = s.dx[c] = s.dy[c] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!in[k]) continue;
+        const float4 tv = tex[(int64_t)cy[k] * W + cx[k]];
+        const float val[3] = {tv.x, tv.y, tv.z};
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            s.t[c] = __fadd_rn(s.t[c], __fmul_rn(val[c], w[k]));
+            if (GRAD) {
+                s.dx[c] += val[c] * wdx[k];
+                s.dy[c] += val[c] * wdy[k];
+            }
+        }
+    }
+    return s;
+}
+
+template <int C>
+__device__ __forceinline__ float tint_mean(const TexSample& s) {
+    return C == 3 ? (s.t[0] + s.t[1] + s.t[2]) / 3.f : s.t[0];
+}
+
+template <int C, bool MOD>
+__global__ __launch_bounds__(TEX_THREADS) void texture_fwd_kernel(float* __restrict__ colors_out,
+                                                                  float* __restrict__ alphas_out,
+                                                                  const float* __restrict__ texcoords,
+                                                                  const float* __restrict__ colors,
+                                                                  const float* __restrict__ alphas,
+                                                                  const float4* __restrict__ tex, int H, int W,
+                                                                  int64_t n, float tr, float one_minus_tr) {
+    const int64_t stride = (int64_t)gridDim.x * TEX_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * TEX_THREADS + threadIdx.x; i < n; i += stride) {
+        const float2 uv = reinterpret_cast<const float2*>(texcoords)[i];
+        const TexSample s = tex_sample<C, false>(tex, H, W, uv.x, uv.y);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float m = __fadd_rn(__fmul_rn(colors[i * 3 + c], tr), one_minus_tr);
+            colors_out[i * 3 + c] = __fmul_rn(m, s.t[C == 3 ? c : 0]);
+        }
+        if (MOD) alphas_out[i] = __fmul_rn(alphas[i], tint_mean<C>(s));
+    }
+}
+
+template <int C, bool MOD, bool GUV>
+__global__ __launch_bounds__(TEX_THREADS) void texture_bwd_kernel(
+    float* __restrict__ g_colors, float* __restrict__ g_alphas, float* __restrict__ g_texcoords,
+    const float* __restrict__ g_colors_out, const float* __restrict__ g_alphas_out,
+    const float* __restrict__ texcoords, const float* __restrict__ colors, const float* __restrict__ alphas,
+    const float4* __restrict__ tex, int H, int W, int64_t n, float tr, float one_minus_tr) {
+    const int64_t stride = (int64_t)gridDim.x * TEX_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * TEX_THREADS + threadIdx.x; i < n; i += stride) {
+        const float2 uv = reinterpret_cast<const float2*>(texcoords)[i];
+        const TexSample s = tex_sample<C, GUV>(tex, H, W, uv.x, uv.y);
+        float gt[3] = {0.f, 0.f, 0.f};  // dL/dtint per channel
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float g = g_colors_out ? g_colors_out[i * 3 + c] : 0.f;
+            g_colors[i * 3 + c] = g * s.t[C == 3 ? c : 0] * tr;
+            if (GUV) {
+                const float m = __fadd_rn(__fmul_rn(colors[i * 3 + c], tr), one_minus_tr);
+                gt[C == 3 ? c : 0] += g * m;
+            }
+        }
+        if (MOD) {
+            const float ga = g_alphas_out ? g_alphas_out[i] : 0.f;
+            g_alphas[i] = ga * tint_mean<C>(s);
+            if (GUV) {
+                const float gm = ga * alphas[i] / (float)C;
+#pragma unroll
+                for (int c = 0; c < C; c++) gt[c] += gm;
+            }
+        }
+        if (GUV) {
+            float gx = 0.f, gy = 0.f;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                gx += gt[c] * s.dx[c];
+                gy += gt[c] * s.dy[c];
+            }
+            // d ix / d u = W - 1 (through u * 2 - 1 and grid_sample's unnormalisation)
+            reinterpret_cast<float2*>(g_texcoords)[i] = make_float2(gx * (float)(W - 1), gy * (float)(H - 1));
+        }
+    }
+}
+
+static int tex_check(const float* texcoords, const float* colors, const float* texture, int64_t height,
+                     int64_t width, int64_t channels, int64_t n) {
+    N3DS_REQUIRE(channels == 1 || channels == 3, "texture should be RGB or grayscale (got %lld channels)",
+                 (long long)channels);
+    N3DS_REQUIRE(height >= 1 && width >= 1 && height * width <= (int64_t(1) << 26),
+                 "texture size %lld x %lld outside [1, 2^26] texels", (long long)height, (long long)width);
+    N3DS_REQUIRE(n >= 0, "negative size");
+    N3DS_REQUIRE(n == 0 || (texcoords && colors && texture), "null input");
+    N3DS_REQUIRE((reinterpret_cast<uintptr_t>(texture) & 15) == 0 && (reinterpret_cast<uintptr_t>(texcoords) & 7) == 0,
+                 "texture must be 16-byte and texcoords 8-byte aligned");
+    return N3DS_OK;
+}
+
+static unsigned tex_blocks(int64_t n) {
+    // enough blocks to keep every CU busy; the grid-stride loop covers the rest
+    return (unsigned)std::min<int64_t>((n + TEX_THREADS - 1) / TEX_THREADS, 256 * 16);
+}
+
+}  // namespace n3ds
+
+using namespace n3ds;
+
+N3DS_EXPORT int n3ds_texture_forward(float* colors_out, float* alphas_out, const float* texcoords, const float* colors,
+                                     const float* alphas, const float* texture, int64_t height, int64_t width,
+                                     int64_t channels, int64_t n, double tint_ratio, int modulate_alpha,
+                                     void* stream) {
+    if (int e = tex_check(texcoords, colors, texture, height, width, channels, n)) return e;
+    N3DS_REQUIRE(n == 0 || colors_out, "null output");
+    N3DS_REQUIRE(!modulate_alpha || n == 0 || (alphas && alphas_out), "modulate_alpha needs alphas and alphas_out");
+    if (n == 0) return N3DS_OK;
+    const float tr = (float)tint_ratio, omt = (float)(1.0 - tint_ratio);
+    const float4* tex = reinterpret_cast<const float4*>(texture);
+    const int H = (int)height, W = (int)width;
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(tex_blocks(n)), dim3(TEX_THREADS), 0, (hipStream_t)stream, colors_out,
+                           alphas_out, texcoords, colors, alphas, tex, H, W, n, tr, omt);
+    };
+    if (channels == 3) modulate_alpha ? launch(texture_fwd_kernel<3, true>) : launch(texture_fwd_kernel<3, false>);
+    else modulate_alpha ? launch(texture_fwd_kernel<1, true>) : launch(texture_fwd_kernel<1, false>);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int n3ds_texture_backward(float* grad_colors, float* grad_alphas, float* grad_texcoords,
+                                      const float* grad_colors_out, const float* grad_alphas_out,
+                                      const float* texcoords, const float* colors, const float* alphas,
+                                      const float* texture, int64_t height, int64_t width, int64_t channels, int64_t n,
+                                      double tint_ratio, int modulate_alpha, void* stream) {
+    if (int e = tex_check(texcoords, colors, texture, height, width, channels, n)) return e;
+    N3DS_REQUIRE(n == 0 || grad_colors, "null grad_colors");
+    N3DS_REQUIRE(!modulate_alpha || n == 0 || (alphas && grad_alphas), "modulate_alpha needs alphas and grad_alphas");
+    N3DS_REQUIRE((reinterpret_cast<uintptr_t>(grad_texcoords) & 7) == 0, "grad_texcoords must be 8-byte aligned");
+    if (n == 0) return N3DS_OK;
+    const float tr = (float)tint_ratio, omt = (float)(1.0 - tint_ratio);
+    const float4* tex = reinterpret_cast<const float4*>(texture);
+    const int H = (int)height, W = (int)width;
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(tex_blocks(n)), dim3(TEX_THREADS), 0, (hipStream_t)stream, grad_colors,
+                           grad_alphas, grad_texcoords, grad_colors_out, grad_alphas_out, texcoords, colors, alphas,
+                           tex, H, W, n, tr, omt);
+    };
+    const bool guv = grad_texcoords != nullptr;
+    if (channels == 3) {
+        if (modulate_alpha) guv ? launch(texture_bwd_kernel<3, true, true>) : launch(texture_bwd_kernel<3, true, false>);
+        else guv ? launch(texture_bwd_kernel<3, false, true>) : launch(texture_bwd_kernel<3, false, false>);
+    } else {
+        if (modulate_alpha) guv ? launch(texture_bwd_kernel<1, true, true>) : launch(texture_bwd_kernel<1, true, false>);
+        else guv ? launch(texture_bwd_kernel<1, false, true>) : launch(texture_bwd_kernel<1, false, false>);
+    }
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}

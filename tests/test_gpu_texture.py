@@ -1,0 +1,156 @@
+"""HIP image texture (n3ds_texture_forward / _backward through ``ImageTexture``)
+against the float64 oracle restatement of textures.py:51-82, against the same
+ops in torch float32 on the GPU, and inside the stroke -> texture -> compose
+chain of models.py:681-709.  Tolerances: outputs 1e-5, gradients 1e-4
+(norm-relative)."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from conftest import rel_err
+from helpers import points_near, stroke_params
+from neural3dstrokes_amd.strokelib import strokes as dstrokes
+from neural3dstrokes_amd.textures import ImageTexture
+from oracle import texture as O
+
+pytestmark = pytest.mark.gpu
+
+OUT_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def _image(seed, h, w, c):
+    g = np.random.default_rng(seed)
+    img = g.integers(0, 256, (h, w, c)).astype(np.float32)
+    return img[..., 0] if c == 1 else img
+
+
+def _pairs(seed, n, lo=-0.1, hi=1.1):
+    g = torch.Generator().manual_seed(seed)
+    tc = torch.rand(n, 2, generator=g) * (hi - lo) + lo
+    tc[:4] = torch.tensor([[0., 0.], [1., 1.], [0., 1.], [1., 0.]])
+    return tc, torch.rand(n, 3, generator=g), torch.rand(n, generator=g)
+
+
+def _torch_ops(image_texture, tc, colors, alphas, tint_ratio, modulate):
+    """textures.py:51-82 with torch ops (float32, same device)."""
+    tint = F.grid_sample(image_texture, (tc.reshape(1, 1, -1, 2) * 2 - 1), align_corners=True)[0, :, 0, :]
+    tint = tint.permute(1, 0)
+    cc = (colors * tint_ratio + (1.0 - tint_ratio)) * tint
+    aa = alphas * tint.mean(-1) if modulate else alphas
+    return cc, aa
+
+
+@pytest.mark.parametrize('channels,tint_ratio,modulate', [(3, 1.0, False), (3, 0.6, True), (1, 0.3, True),
+                                                          (1, 1.0, False)])
+def test_texture_matches_oracle(channels, tint_ratio, modulate, dev):
+    tex = ImageTexture(_image(channels, 96, 128, channels), (48, 64), tint_ratio=tint_ratio,
+                       modulate_alpha=modulate).to(dev)
+    n = 40000
+    tc, col, al = _pairs(channels + 10, n)
+    leaves = [t.to(dev).requires_grad_(True) for t in (tc, col, al)]
+    cc, aa = tex(leaves[0].reshape(200, 200, 2), leaves[1].reshape(200, 200, 3), leaves[2].reshape(200, 200))
+    assert cc.shape == (200, 200, 3) and aa.shape == (200, 200)
+    g = torch.Generator().manual_seed(4)
+    gc, ga = torch.randn(n, 3, generator=g), torch.randn(n, generator=g)
+    grads = torch.autograd.grad([cc, aa], leaves, [gc.to(dev).reshape(cc.shape), ga.to(dev).reshape(aa.shape)],
+                                allow_unused=True)
+    image = tex.image_texture[0].double().cpu().numpy()
+    d = [t.double().numpy() for t in (tc, col, al)]
+    oc, oa = O.texture_forward(image, *d, tint_ratio, modulate)
+    assert rel_err(cc.detach().reshape(-1, 3), torch.from_numpy(oc)) < OUT_TOL
+    # float32 pixel coordinates u * (W - 1) carry ~W * 6e-8 px of rounding (as in grid_sample)
+    assert float((cc.detach().reshape(-1, 3).double().cpu() - torch.from_numpy(oc)).abs().max()) < OUT_TOL
+    assert rel_err(aa.detach().reshape(-1), torch.from_numpy(oa)) < OUT_TOL
+    d_col, d_al, d_tc = O.texture_backward(image, *d, gc.double().numpy(), ga.double().numpy(), tint_ratio, modulate)
+    for name, a, b in zip(('colors', 'alphas'), grads[1:], (d_col, d_al)):
+        assert rel_err(a, torch.from_numpy(b)) < GRAD_TOL, name
+    # d tint / d texcoord jumps at texel boundaries, and the float32 pixel coordinate
+    # (rounded ~2e-6 px, as in grid_sample) may fall on the other side of one than the
+    # float64 coordinate: compare the pairs more than 1e-4 px from a boundary
+    px = d[0] * np.array([image.shape[2] - 1, image.shape[1] - 1])
+    frac = px - np.floor(px)
+    keep = torch.from_numpy(((frac > 1e-4) & (frac < 1 - 1e-4)).all(1))
+    assert float(keep.double().mean()) > 0.999
+    assert rel_err(grads[0].cpu()[keep], torch.from_numpy(d_tc)[keep]) < GRAD_TOL, 'texcoords'
+
+
+@pytest.mark.parametrize('channels,modulate', [(3, True), (1, False)])
+def test_texture_matches_torch_ops(channels, modulate, dev):
+    """Same outputs and gradients as the reference module's own torch ops in float32."""
+    tex = ImageTexture(_image(5, 64, 64, channels), (64, 64), tint_ratio=0.8, modulate_alpha=modulate).to(dev)
+    tc, col, al = _pairs(9, 30000)
+    a_leaves = [t.to(dev).requires_grad_(True) for t in (tc, col, al)]
+    b_leaves = [t.to(dev).requires_grad_(True) for t in (tc, col, al)]
+    cc, aa = tex(*a_leaves)
+    rc, ra = _torch_ops(tex.image_texture, *b_leaves, 0.8, modulate)
+    # torch's float32 pixel coordinate may round differently from ours by ~1e-6 px
+    assert float((cc - rc).detach().abs().max()) < OUT_TOL and float((aa - ra).detach().abs().max()) < OUT_TOL
+    g = torch.Generator().manual_seed(2)
+    ups = [torch.randn(cc.shape, generator=g).to(dev), torch.randn(aa.shape, generator=g).to(dev)]
+    ga = torch.autograd.grad([cc, aa], a_leaves, ups)
+    gb = torch.autograd.grad([rc, ra], b_leaves, ups)
+    for a, b in zip(ga[1:], gb[1:]):
+        assert rel_err(a, b) < GRAD_TOL
+    # the texcoord gradient jumps at texel boundaries: compare pairs more than 1e-4 px away
+    px = tc.double() * torch.tensor([63.0, 63.0], dtype=torch.float64)
+    frac = px - px.floor()
+    keep = ((frac > 1e-4) & (frac < 1 - 1e-4)).all(1)
+    assert float(keep.double().mean()) > 0.999
+    assert rel_err(ga[0].cpu()[keep], gb[0].cpu()[keep]) < GRAD_TOL
+    s = tex.sample_texture(a_leaves[0].detach())
+    rs = F.grid_sample(tex.image_texture, a_leaves[0].detach().reshape(1, 1, -1, 2) * 2 - 1,
+                       align_corners=True)[0, :, 0, :].permute(1, 0)
+    assert s.shape == rs.shape and float((s - rs).abs().max()) < OUT_TOL
+
+
+def test_texture_in_stroke_chain(dev):
+    """get_stroke(return_texcoord) -> ImageTexture -> compose_strokes (models.py:681-709): the
+    shape / colour / density gradients equal those of the chain with the torch texture ops
+    (the stroke backward drops the texcoord gradient, as in the reference)."""
+    shape = 'ellipsoid'
+    sp, cp = stroke_params(shape, 'constant_rgb', 8, 3)
+    x, vd = points_near(shape, sp, 64, 32, 3)
+    r = torch.full((64, 32), 0.05)
+    dp = torch.rand(8, generator=torch.Generator().manual_seed(1)) * 10 + 1
+    tex = ImageTexture(_image(8, 32, 32, 3), (32, 32), modulate_alpha=True, tint_ratio=0.7).to(dev)
+    fn = dstrokes.get_stroke(shape, 'constant_rgb', 'recon')[0]
+    outs = []
+    for use_kernel in (True, False):
+        leaves = [t.to(dev).requires_grad_(True) for t in (sp, cp, dp)]
+        a, c, _, t = fn(x.to(dev), r.to(dev), vd.to(dev), leaves[0], leaves[1], 1.0, False, False, True, True)
+        assert t.shape == (64, 32, 8, 2)
+        if use_kernel:
+            c, a = tex(t, c, a)
+        else:
+            cc, aa = _torch_ops(tex.image_texture, t.reshape(-1, 2), c.reshape(-1, 3), a.reshape(-1), 0.7, True)
+            c, a = cc.reshape(c.shape), aa.reshape(a.shape)
+        dens, col = dstrokes.compose_strokes(a, c, leaves[2], 'over')
+        g = torch.Generator().manual_seed(6)
+        loss = (dens * torch.randn(dens.shape, generator=g).to(dev)).sum() + \
+            (col * torch.randn(col.shape, generator=g).to(dev)).sum()
+        outs.append((dens.detach(), col.detach()) + torch.autograd.grad(loss, leaves))
+    for a, b in zip(outs[0], outs[1]):
+        assert rel_err(a, b) < GRAD_TOL
+
+
+def test_texture_edge_cases(dev):
+    tex = ImageTexture(_image(1, 16, 16, 3), (16, 16), modulate_alpha=True).to(dev)
+    # no pairs
+    z = torch.zeros(0, 2, device=dev)
+    cc, aa = tex(z, torch.zeros(0, 3, device=dev), torch.zeros(0, device=dev))
+    assert cc.shape == (0, 3) and aa.shape == (0,)
+    # far outside and NaN texcoords sample zero
+    tc = torch.tensor([[-1.0, 0.5], [0.5, 3.0], [float('nan'), 0.5], [1e30, -1e30]], device=dev)
+    cc, aa = tex(tc, torch.ones(4, 3, device=dev), torch.ones(4, device=dev))
+    assert torch.equal(cc, torch.zeros_like(cc)) and torch.equal(aa, torch.zeros_like(aa))
+    # a 1x1 texture is its texel everywhere
+    one = ImageTexture(np.full((1, 1, 3), 51.0, dtype=np.float32), (1, 1)).to(dev)
+    tc, col, al = _pairs(3, 100)
+    cc, _ = one(tc.to(dev), col.to(dev), al.to(dev))
+    assert float((cc.cpu() - col * 0.2).abs().max()) < 1e-6
+    # an RGBA image is refused, as in the reference
+    rgba = ImageTexture(np.zeros((4, 4, 4), dtype=np.float32), (4, 4)).to(dev)
+    with pytest.raises(AssertionError):
+        rgba(tc.to(dev), col.to(dev), al.to(dev))

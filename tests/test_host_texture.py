@@ -1,0 +1,61 @@
+"""Host side of the image texture (no device): image set-up as in textures.py:36-49,
+the device texel layout, the config factory, and argument validation of the C-ABI."""
+import ctypes
+import types
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from neural3dstrokes_amd import _lib
+from neural3dstrokes_amd.textures import ImageTexture, get_stroke_texture
+
+
+def _png(tmp_path, arr):
+    from PIL import Image
+    p = tmp_path / 'tex.png'
+    Image.fromarray(arr).save(p)
+    return str(p)
+
+
+def test_image_setup_matches_reference_steps(tmp_path):
+    g = np.random.default_rng(0)
+    arr = g.integers(0, 256, (40, 30, 3)).astype(np.uint8)
+    tex = ImageTexture(_png(tmp_path, arr), (20, 15), tint_ratio=0.5)
+    want = F.interpolate((torch.from_numpy(arr.astype(np.float32)) / 255.).permute(2, 0, 1)[None], (20, 15),
+                         mode='area')
+    assert torch.equal(tex.image_texture, want)
+    assert tex.texels.shape == (20, 15, 4) and tex.channels == 3 and tex.tint_ratio == 0.5
+    assert torch.equal(tex.texels[..., :3], want[0].permute(1, 2, 0)) and not tex.texels[..., 3].any()
+    # a grey image keeps one channel; an array gives the same buffers as the file
+    grey = g.integers(0, 256, (8, 8)).astype(np.uint8)
+    t1, t2 = ImageTexture(_png(tmp_path, grey), (8, 8)), ImageTexture(grey.astype(np.float32), (8, 8))
+    assert t1.channels == 1 and torch.equal(t1.image_texture, t2.image_texture)
+    assert torch.equal(t1.texels, t2.texels)
+    assert tex.modulate_alpha is False and ImageTexture(grey, (4, 4), modulate_alpha=True).modulate_alpha
+
+
+def test_stroke_texture_factory(tmp_path):
+    path = _png(tmp_path, np.zeros((4, 4, 3), dtype=np.uint8))
+    assert get_stroke_texture(types.SimpleNamespace(stroke_texture='none')) is None
+    t = get_stroke_texture(types.SimpleNamespace(stroke_texture='image', texture_image_path=path,
+                                                 texture_image_size=(2, 2)))
+    assert isinstance(t, ImageTexture) and t.image_texture.shape == (1, 3, 2, 2)
+    with pytest.raises(AssertionError):
+        get_stroke_texture(types.SimpleNamespace(stroke_texture='video'))
+
+
+def test_texture_rejects_cpu_tensors_and_bad_arguments():
+    tex = ImageTexture(np.zeros((4, 4, 3), dtype=np.float32), (4, 4))
+    with pytest.raises(RuntimeError, match='CUDA'):
+        tex(torch.rand(5, 2), torch.rand(5, 3), torch.rand(5))
+    lib = _lib.lib()
+    # validation fails before any device work
+    assert lib.n3ds_texture_forward(None, None, None, None, None, None, 4, 4, 2, 0, 1.0, 0, None) == 1
+    assert b'RGB or grayscale' in lib.n3ds_last_error()
+    assert lib.n3ds_texture_forward(None, None, None, None, None, None, 0, 4, 3, 0, 1.0, 0, None) == 1
+    assert lib.n3ds_texture_backward(None, None, None, None, None, None, None, None, None, 4, 4, 3, -1, 1.0, 0,
+                                     None) == 1
+    assert lib.n3ds_texture_forward(None, None, None, None, None, ctypes.c_void_p(8), 4, 4, 3, 5, 1.0, 0,
+                                    None) == 1

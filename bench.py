@@ -254,15 +254,16 @@ def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
     g = torch.Generator().manual_seed(seed)
     img = (torch.rand(256, 256, 3, generator=g) * 255).round().numpy()
     tex = ImageTexture(img, (256, 256), tint_ratio=0.7, modulate_alpha=True).to(dev)
-    tc = torch.rand(n_pairs, 2, device=dev)
-    col = torch.rand(n_pairs, 3, device=dev)
-    al = torch.rand(n_pairs, device=dev)
-    gc = torch.randn(n_pairs, 3, device=dev)
-    ga = torch.randn(n_pairs, device=dev)
+    gd = torch.Generator(device=dev).manual_seed(seed)
+    tc = torch.rand(n_pairs, 2, device=dev, generator=gd)
+    col = torch.rand(n_pairs, 3, device=dev, generator=gd)
+    al = torch.rand(n_pairs, device=dev, generator=gd)
+    gc = torch.randn(n_pairs, 3, device=dev, generator=gd)
+    ga = torch.randn(n_pairs, device=dev, generator=gd)
     col_o, al_o = torch.empty_like(col), torch.empty_like(al)
     g_col, g_al, g_tc = torch.empty_like(col), torch.empty_like(al), torch.empty_like(tc)
     lib = _lib.lib()
-    T = tex.texels
+    T = tex.packed()
 
     def fwd():
         _lib.check(lib.n3ds_texture_forward(_lib.ptr(col_o), _lib.ptr(al_o), _lib.ptr(tc), _lib.ptr(col), _lib.ptr(al),

@@ -324,14 +324,24 @@ int n3ds_adam_step(float* params, float* grads, float* exp_avg, float* exp_avg_s
  *                      align_corners=True);
  *   colors_out = (colors * tint_ratio + (1 - tint_ratio)) * tint;
  *   alphas_out = alphas * mean(tint)   (only when modulate_alpha).
- * texture [H, W, 4] float (channel c of texel (y, x) at (y*W + x)*4 + c,
- * 16-byte aligned; the first `channels` in {1, 3} are used; H*W <= 2^26),
+ * quads: the image packed by n3ds_texture_pack (16-byte aligned),
  * texcoords [N, 2] (u along W, v along H), colors / colors_out [N, 3],
  * alphas / alphas_out [N] (read / written only when modulate_alpha).
  * ------------------------------------------------------------------------ */
 int n3ds_texture_forward(float* colors_out, float* alphas_out, const float* texcoords, const float* colors,
-                         const float* alphas, const float* texture, int64_t height, int64_t width, int64_t channels,
+                         const float* alphas, const float* quads, int64_t height, int64_t width, int64_t channels,
                          int64_t n, double tint_ratio, int modulate_alpha, void* stream);
+
+/* Floats of the packed form of an H x W image with `channels` in {1, 3}
+ * ((H+1)(W+1) entries of 16 or 4 floats; H*W <= 2^26), or -1. */
+int64_t n3ds_texture_pack_floats(int64_t height, int64_t width, int64_t channels);
+
+/* Packs image [C, H, W] float (the reference's image_texture[0],
+ * textures.py:45-49) into quads: entry (j, i) holds the corners
+ * (x, y) = (i-1, j-1), (i, j-1), (i-1, j), (i, j) channel by channel, 0 off
+ * the image, so one bilinear tap reads one aligned entry. */
+int n3ds_texture_pack(float* quads, const float* image, int64_t height, int64_t width, int64_t channels,
+                      void* stream);
 
 /* Adjoint of n3ds_texture_forward: what torch's autograd gives the ops of
  * textures.py:51-82 (the image is a non-persistent buffer without gradient).
@@ -341,7 +351,7 @@ int n3ds_texture_forward(float* colors_out, float* alphas_out, const float* texc
  * grad_alphas_out may be NULL (zeros). */
 int n3ds_texture_backward(float* grad_colors, float* grad_alphas, float* grad_texcoords,
                           const float* grad_colors_out, const float* grad_alphas_out, const float* texcoords,
-                          const float* colors, const float* alphas, const float* texture, int64_t height,
+                          const float* colors, const float* alphas, const float* quads, int64_t height,
                           int64_t width, int64_t channels, int64_t n, double tint_ratio, int modulate_alpha,
                           void* stream);
 

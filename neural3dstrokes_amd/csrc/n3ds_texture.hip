@@ -5,12 +5,18 @@
 //   colour = (colour * tint_ratio + (1 - tint_ratio)) * tint        (:77-78)
 //   alpha  = alpha * mean(tint)          when modulate_alpha        (:79-80)
 //
-// Layout: the texture sits in HBM as [H][W] float4 (channels 0..C-1 used,
-// C = 1 or 3), so one corner is one 16-byte gather; a 256x256 texture is
-// 1 MiB and stays resident in each XCD's L2.  One thread per (sample, stroke)
-// pair in a grid-stride loop; per pair the kernel moves 8 B of texcoords,
-// 12 B of colour in and out, and 4 B of alpha in and out when modulating, so
-// it is HBM bound at large N.
+// Layout: the image [C][H][W] (the reference's buffer) is packed once into
+// "quads": for every cell origin (y0, x0), y0 in [-1, H-1], x0 in [-1, W-1],
+// one entry holds the four corners nw (x0,y0), ne (x0+1,y0), sw (x0,y0+1),
+// se (x0+1,y0+1) channel by channel, with zeros for corners off the image
+// (zero padding).  An RGB entry is 12 floats padded to 64 B, a grey entry 4
+// floats (16 B), so a bilinear tap reads one aligned segment of one cache
+// line instead of four scattered texels; a 256x256 RGB texture packs to
+// 4.2 MB, resident in L2 / MALL.  A stored zero times a finite weight adds
+// +0, so the result equals skipping the off-image corners.  One thread per
+// (sample, stroke) pair in a grid-stride loop; per pair the kernel streams
+// 8 B of texcoords, 12 B of colour in and out, and 4 B of alpha in and out
+// when modulating.
 //
 // Derivatives are those torch's autograd gives the same ops (the image is a
 // buffer without gradient): d colour_in = g * tint * tint_ratio,
@@ -30,10 +36,14 @@ struct TexSample {
     float dx[3], dy[3];  // derivatives w.r.t. the unnormalised pixel coordinates ix, iy
 };
 
-// Bilinear tap of an [H][W] float4 texture at texcoords (u along W, v along H)
-// in [0, 1]; corners outside the texture contribute zero.
+// floats per quad entry
+template <int C>
+constexpr int quad_floats() { return C == 3 ? 16 : 4; }
+
+// Bilinear tap of a packed quad texture at texcoords (u along W, v along H)
+// in [0, 1]; corners outside the image contribute zero.
 template <int C, bool GRAD>
-__device__ __forceinline__ TexSample tex_sample(const float4* __restrict__ tex, int H, int W, float u, float v) {
+__device__ __forceinline__ TexSample tex_sample(const float4* __restrict__ quads, int H, int W, float u, float v) {
     // the reference maps [0, 1] to [-1, 1] (uvs = texcoords * 2 - 1) and
     // grid_sample maps that back to pixels ((g + 1) / 2) * (size - 1)
     const float gu = __fsub_rn(__fmul_rn(u, 2.f), 1.f), gv = __fsub_rn(__fmul_rn(v, 2.f), 1.f);
@@ -43,33 +53,64 @@ __device__ __forceinline__ TexSample tex_sample(const float4* __restrict__ tex, 
     // and keeps the integer corner indices far from overflow
     const float fx = fminf(fmaxf(floorf(ix), -2.f), (float)W), fy = fminf(fmaxf(floorf(iy), -2.f), (float)H);
     const int x0 = (int)fx, y0 = (int)fy;
-    const float ex = __fsub_rn(fx + 1.f, ix), ox = __fsub_rn(ix, fx);  // x1 - ix, ix - x0
-    const float ey = __fsub_rn(fy + 1.f, iy), oy = __fsub_rn(iy, fy);
-    const bool inx0 = x0 >= 0 && x0 < W, inx1 = x0 + 1 >= 0 && x0 + 1 < W;
-    const bool iny0 = y0 >= 0 && y0 < H, iny1 = y0 + 1 >= 0 && y0 + 1 < H;
-    // corner order nw (x0,y0), ne (x1,y0), sw (x0,y1), se (x1,y1)
-    const bool in[4] = {inx0 && iny0, inx1 && iny0, inx0 && iny1, inx1 && iny1};
-    const int cx[4] = {x0, x0 + 1, x0, x0 + 1}, cy[4] = {y0, y0, y0 + 1, y0 + 1};
-    const float w[4] = {__fmul_rn(ex, ey), __fmul_rn(ox, ey), __fmul_rn(ex, oy), __fmul_rn(ox, oy)};
-    const float wdx[4] = {-ey, ey, -oy, oy}, wdy[4] = {-ex, -ox, ex, ox};
     TexSample s;
 #pragma unroll
     for (int c = 0; c < 3; c++) s.t[c] = s.dx[c] = s.dy[c] = 0.f;
+    if (x0 < -1 || x0 > W - 1 || y0 < -1 || y0 > H - 1) return s;  // no corner on the image
+    const float ex = __fsub_rn(fx + 1.f, ix), ox = __fsub_rn(ix, fx);  // x1 - ix, ix - x0
+    const float ey = __fsub_rn(fy + 1.f, iy), oy = __fsub_rn(iy, fy);
+    // corner order nw, ne, sw, se
+    const float w[4] = {__fmul_rn(ex, ey), __fmul_rn(ox, ey), __fmul_rn(ex, oy), __fmul_rn(ox, oy)};
+    const float wdx[4] = {-ey, ey, -oy, oy}, wdy[4] = {-ex, -ox, ex, ox};
+    const float4* q = quads + ((int64_t)(y0 + 1) * (W + 1) + (x0 + 1)) * (quad_floats<C>() / 4);
+    float val[4 * C];
+    if (C == 3) {
+        const float4 a = q[0], b = q[1], c = q[2];
+        const float t[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int k = 0; k < 4 * C; k++) val[k] = t[k];
+    } else {
+        const float4 a = q[0];
+        const float t[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int k = 0; k < 4 * C; k++) val[k] = t[k];
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (!in[k]) continue;
-        const float4 tv = tex[(int64_t)cy[k] * W + cx[k]];
-        const float val[3] = {tv.x, tv.y, tv.z};
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            s.t[c] = __fadd_rn(s.t[c], __fmul_rn(val[c], w[k]));
+            s.t[c] = __fadd_rn(s.t[c], __fmul_rn(val[k * C + c], w[k]));
             if (GRAD) {
-                s.dx[c] += val[c] * wdx[k];
-                s.dy[c] += val[c] * wdy[k];
+                s.dx[c] += val[k * C + c] * wdx[k];
+                s.dy[c] += val[k * C + c] * wdy[k];
             }
         }
     }
     return s;
+}
+
+// quads[(j * (W + 1) + i) * F + k * C + c] = image[c][j - 1 + dy_k][i - 1 + dx_k] or 0 off the image
+template <int C>
+__global__ __launch_bounds__(TEX_THREADS) void texture_pack_kernel(float* __restrict__ quads,
+                                                                   const float* __restrict__ image, int H, int W) {
+    constexpr int F = quad_floats<C>();
+    const int64_t n = (int64_t)(H + 1) * (W + 1);
+    for (int64_t e = (int64_t)blockIdx.x * TEX_THREADS + threadIdx.x; e < n; e += (int64_t)gridDim.x * TEX_THREADS) {
+        const int j = (int)(e / (W + 1)), i = (int)(e % (W + 1));
+        float out[F];
+#pragma unroll
+        for (int k = 0; k < F; k++) out[k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int x = i - 1 + (k & 1), y = j - 1 + (k >> 1);
+            if (x < 0 || x >= W || y < 0 || y >= H) continue;
+#pragma unroll
+            for (int c = 0; c < C; c++) out[k * C + c] = image[((int64_t)c * H + y) * W + x];
+        }
+        float4* dst = reinterpret_cast<float4*>(quads + e * F);
+#pragma unroll
+        for (int k = 0; k < F / 4; k++) dst[k] = make_float4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+    }
 }
 
 template <int C>
@@ -83,12 +124,12 @@ __global__ __launch_bounds__(TEX_THREADS) void texture_fwd_kernel(float* __restr
                                                                   const float* __restrict__ texcoords,
                                                                   const float* __restrict__ colors,
                                                                   const float* __restrict__ alphas,
-                                                                  const float4* __restrict__ tex, int H, int W,
+                                                                  const float4* __restrict__ quads, int H, int W,
                                                                   int64_t n, float tr, float one_minus_tr) {
     const int64_t stride = (int64_t)gridDim.x * TEX_THREADS;
     for (int64_t i = (int64_t)blockIdx.x * TEX_THREADS + threadIdx.x; i < n; i += stride) {
         const float2 uv = reinterpret_cast<const float2*>(texcoords)[i];
-        const TexSample s = tex_sample<C, false>(tex, H, W, uv.x, uv.y);
+        const TexSample s = tex_sample<C, false>(quads, H, W, uv.x, uv.y);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const float m = __fadd_rn(__fmul_rn(colors[i * 3 + c], tr), one_minus_tr);
@@ -103,11 +144,11 @@ __global__ __launch_bounds__(TEX_THREADS) void texture_bwd_kernel(
     float* __restrict__ g_colors, float* __restrict__ g_alphas, float* __restrict__ g_texcoords,
     const float* __restrict__ g_colors_out, const float* __restrict__ g_alphas_out,
     const float* __restrict__ texcoords, const float* __restrict__ colors, const float* __restrict__ alphas,
-    const float4* __restrict__ tex, int H, int W, int64_t n, float tr, float one_minus_tr) {
+    const float4* __restrict__ quads, int H, int W, int64_t n, float tr, float one_minus_tr) {
     const int64_t stride = (int64_t)gridDim.x * TEX_THREADS;
     for (int64_t i = (int64_t)blockIdx.x * TEX_THREADS + threadIdx.x; i < n; i += stride) {
         const float2 uv = reinterpret_cast<const float2*>(texcoords)[i];
-        const TexSample s = tex_sample<C, GUV>(tex, H, W, uv.x, uv.y);
+        const TexSample s = tex_sample<C, GUV>(quads, H, W, uv.x, uv.y);
         float gt[3] = {0.f, 0.f, 0.f};  // dL/dtint per channel
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -140,16 +181,16 @@ __global__ __launch_bounds__(TEX_THREADS) void texture_bwd_kernel(
     }
 }
 
-static int tex_check(const float* texcoords, const float* colors, const float* texture, int64_t height,
+static int tex_check(const float* texcoords, const float* colors, const float* quads, int64_t height,
                      int64_t width, int64_t channels, int64_t n) {
     N3DS_REQUIRE(channels == 1 || channels == 3, "texture should be RGB or grayscale (got %lld channels)",
                  (long long)channels);
     N3DS_REQUIRE(height >= 1 && width >= 1 && height * width <= (int64_t(1) << 26),
                  "texture size %lld x %lld outside [1, 2^26] texels", (long long)height, (long long)width);
     N3DS_REQUIRE(n >= 0, "negative size");
-    N3DS_REQUIRE(n == 0 || (texcoords && colors && texture), "null input");
-    N3DS_REQUIRE((reinterpret_cast<uintptr_t>(texture) & 15) == 0 && (reinterpret_cast<uintptr_t>(texcoords) & 7) == 0,
-                 "texture must be 16-byte and texcoords 8-byte aligned");
+    N3DS_REQUIRE(n == 0 || (texcoords && colors && quads), "null input");
+    N3DS_REQUIRE((reinterpret_cast<uintptr_t>(quads) & 15) == 0 && (reinterpret_cast<uintptr_t>(texcoords) & 7) == 0,
+                 "packed texture must be 16-byte and texcoords 8-byte aligned");
     return N3DS_OK;
 }
 
@@ -162,20 +203,49 @@ static unsigned tex_blocks(int64_t n) {
 
 using namespace n3ds;
 
+N3DS_EXPORT int64_t n3ds_texture_pack_floats(int64_t height, int64_t width, int64_t channels) {
+    if (channels != 1 && channels != 3) {
+        set_error("texture should be RGB or grayscale (got %lld channels)", (long long)channels);
+        return -1;
+    }
+    if (height < 1 || width < 1 || height * width > (int64_t(1) << 26)) {
+        set_error("texture size %lld x %lld outside [1, 2^26] texels", (long long)height, (long long)width);
+        return -1;
+    }
+    return (height + 1) * (width + 1) * (channels == 3 ? 16 : 4);
+}
+
+N3DS_EXPORT int n3ds_texture_pack(float* quads, const float* image, int64_t height, int64_t width, int64_t channels,
+                                  void* stream) {
+    if (n3ds_texture_pack_floats(height, width, channels) < 0) return N3DS_EINVAL;  // error already set
+    N3DS_REQUIRE(quads && image, "null argument");
+    N3DS_REQUIRE((reinterpret_cast<uintptr_t>(quads) & 15) == 0, "packed texture must be 16-byte aligned");
+    const int H = (int)height, W = (int)width;
+    const unsigned blocks = tex_blocks((height + 1) * (width + 1));
+    if (channels == 3)
+        hipLaunchKernelGGL(texture_pack_kernel<3>, dim3(blocks), dim3(TEX_THREADS), 0, (hipStream_t)stream, quads,
+                           image, H, W);
+    else
+        hipLaunchKernelGGL(texture_pack_kernel<1>, dim3(blocks), dim3(TEX_THREADS), 0, (hipStream_t)stream, quads,
+                           image, H, W);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
 N3DS_EXPORT int n3ds_texture_forward(float* colors_out, float* alphas_out, const float* texcoords, const float* colors,
-                                     const float* alphas, const float* texture, int64_t height, int64_t width,
+                                     const float* alphas, const float* quads, int64_t height, int64_t width,
                                      int64_t channels, int64_t n, double tint_ratio, int modulate_alpha,
                                      void* stream) {
-    if (int e = tex_check(texcoords, colors, texture, height, width, channels, n)) return e;
+    if (int e = tex_check(texcoords, colors, quads, height, width, channels, n)) return e;
     N3DS_REQUIRE(n == 0 || colors_out, "null output");
     N3DS_REQUIRE(!modulate_alpha || n == 0 || (alphas && alphas_out), "modulate_alpha needs alphas and alphas_out");
     if (n == 0) return N3DS_OK;
     const float tr = (float)tint_ratio, omt = (float)(1.0 - tint_ratio);
-    const float4* tex = reinterpret_cast<const float4*>(texture);
+    const float4* q = reinterpret_cast<const float4*>(quads);
     const int H = (int)height, W = (int)width;
     auto launch = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(tex_blocks(n)), dim3(TEX_THREADS), 0, (hipStream_t)stream, colors_out,
-                           alphas_out, texcoords, colors, alphas, tex, H, W, n, tr, omt);
+                           alphas_out, texcoords, colors, alphas, q, H, W, n, tr, omt);
     };
     if (channels == 3) modulate_alpha ? launch(texture_fwd_kernel<3, true>) : launch(texture_fwd_kernel<3, false>);
     else modulate_alpha ? launch(texture_fwd_kernel<1, true>) : launch(texture_fwd_kernel<1, false>);
@@ -186,20 +256,20 @@ N3DS_EXPORT int n3ds_texture_forward(float* colors_out, float* alphas_out, const
 N3DS_EXPORT int n3ds_texture_backward(float* grad_colors, float* grad_alphas, float* grad_texcoords,
                                       const float* grad_colors_out, const float* grad_alphas_out,
                                       const float* texcoords, const float* colors, const float* alphas,
-                                      const float* texture, int64_t height, int64_t width, int64_t channels, int64_t n,
+                                      const float* quads, int64_t height, int64_t width, int64_t channels, int64_t n,
                                       double tint_ratio, int modulate_alpha, void* stream) {
-    if (int e = tex_check(texcoords, colors, texture, height, width, channels, n)) return e;
+    if (int e = tex_check(texcoords, colors, quads, height, width, channels, n)) return e;
     N3DS_REQUIRE(n == 0 || grad_colors, "null grad_colors");
     N3DS_REQUIRE(!modulate_alpha || n == 0 || (alphas && grad_alphas), "modulate_alpha needs alphas and grad_alphas");
     N3DS_REQUIRE((reinterpret_cast<uintptr_t>(grad_texcoords) & 7) == 0, "grad_texcoords must be 8-byte aligned");
     if (n == 0) return N3DS_OK;
     const float tr = (float)tint_ratio, omt = (float)(1.0 - tint_ratio);
-    const float4* tex = reinterpret_cast<const float4*>(texture);
+    const float4* q = reinterpret_cast<const float4*>(quads);
     const int H = (int)height, W = (int)width;
     auto launch = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(tex_blocks(n)), dim3(TEX_THREADS), 0, (hipStream_t)stream, grad_colors,
                            grad_alphas, grad_texcoords, grad_colors_out, grad_alphas_out, texcoords, colors, alphas,
-                           tex, H, W, n, tr, omt);
+                           q, H, W, n, tr, omt);
     };
     const bool guv = grad_texcoords != nullptr;
     if (channels == 3) {

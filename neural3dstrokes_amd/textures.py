@@ -5,9 +5,10 @@ sample of an image at each (sample, stroke) texture coordinate that the stroke
 operator returns with ``return_texcoord=True``; the reference applies it between
 the stroke and compositing operators (``models.py:704-706``).  The image is set up
 as in the reference (loaded as float32, divided by 255, ``F.interpolate(mode='area')``
-to ``texture_size``; ``textures.py:45-49``) and kept on the device as an
-``[H, W, 4]`` texel array for the gfx950 kernels ``n3ds_texture_forward`` /
-``n3ds_texture_backward``.  As in the reference, the image has no gradient.
+to ``texture_size``; ``textures.py:45-49``); on first use on a device it is packed
+by ``n3ds_texture_pack`` into the layout of the gfx950 kernels ``n3ds_texture_forward``
+/ ``n3ds_texture_backward`` (one entry per bilinear cell with its four corners).  As in
+the reference, the image has no gradient.
 """
 import numpy as np
 import torch
@@ -47,14 +48,14 @@ def _aligned(t, align):
     return t if t.data_ptr() % align == 0 else t.clone()
 
 
-def _texture_forward(texels, channels, texcoords, colors, alphas, tint_ratio, modulate_alpha):
-    for name, t in (('texcoords', texcoords), ('colors', colors), ('alphas', alphas), ('texture', texels)):
+def _texture_forward(quads, hw, channels, texcoords, colors, alphas, tint_ratio, modulate_alpha):
+    for name, t in (('texcoords', texcoords), ('colors', colors), ('alphas', alphas), ('texture', quads)):
         _lib.require_cuda(name, t)
     colors_out = torch.empty_like(colors)
     alphas_out = torch.empty_like(alphas) if modulate_alpha else None
     _lib.check(_lib.lib().n3ds_texture_forward(
         _lib.ptr(colors_out), _lib.ptr(alphas_out), _lib.ptr(texcoords), _lib.ptr(colors),
-        _lib.ptr(alphas) if modulate_alpha else None, _lib.ptr(texels), texels.shape[0], texels.shape[1], channels,
+        _lib.ptr(alphas) if modulate_alpha else None, _lib.ptr(quads), hw[0], hw[1], channels,
         colors.shape[0], float(tint_ratio), int(bool(modulate_alpha)), _lib.stream()))
     return colors_out, alphas_out
 
@@ -64,14 +65,14 @@ class _texture_fn(Function):
 
     @staticmethod
     @torch.amp.custom_fwd(device_type='cuda')
-    def forward(ctx, texcoords, colors, alphas, texels, channels, tint_ratio, modulate_alpha):
+    def forward(ctx, texcoords, colors, alphas, quads, hw, channels, tint_ratio, modulate_alpha):
         texcoords = _aligned(texcoords.float(), 8)
         colors = colors.contiguous().float()
         alphas = alphas.contiguous().float()
-        colors_out, alphas_out = _texture_forward(texels, channels, texcoords, colors, alphas, tint_ratio,
+        colors_out, alphas_out = _texture_forward(quads, hw, channels, texcoords, colors, alphas, tint_ratio,
                                                   modulate_alpha)
-        ctx.save_for_backward(texcoords, colors, alphas, texels)
-        ctx.cfg = (channels, tint_ratio, modulate_alpha)
+        ctx.save_for_backward(texcoords, colors, alphas, quads)
+        ctx.cfg = (hw, channels, tint_ratio, modulate_alpha)
         if alphas_out is None:
             # alphas pass through unchanged (textures.py:79-80): the caller keeps its own tensor
             alphas_out = torch.zeros(0, dtype=torch.float32, device=colors.device)
@@ -82,8 +83,8 @@ class _texture_fn(Function):
     @once_differentiable
     @torch.amp.custom_bwd(device_type='cuda')
     def backward(ctx, grad_colors_out, grad_alphas_out):
-        texcoords, colors, alphas, texels = ctx.saved_tensors
-        channels, tint_ratio, modulate_alpha = ctx.cfg
+        texcoords, colors, alphas, quads = ctx.saved_tensors
+        hw, channels, tint_ratio, modulate_alpha = ctx.cfg
         need_uv = ctx.needs_input_grad[0]
         gc_out = None if grad_colors_out is None else grad_colors_out.contiguous().float()
         ga_out = None if (grad_alphas_out is None or not modulate_alpha) else grad_alphas_out.contiguous().float()
@@ -92,9 +93,9 @@ class _texture_fn(Function):
         g_uv = torch.empty_like(texcoords) if need_uv else None
         _lib.check(_lib.lib().n3ds_texture_backward(
             _lib.ptr(g_colors), _lib.ptr(g_alphas), _lib.ptr(g_uv), _lib.ptr(gc_out), _lib.ptr(ga_out),
-            _lib.ptr(texcoords), _lib.ptr(colors), _lib.ptr(alphas), _lib.ptr(texels), texels.shape[0],
-            texels.shape[1], channels, colors.shape[0], float(tint_ratio), int(bool(modulate_alpha)), _lib.stream()))
-        return g_uv, g_colors, g_alphas, None, None, None, None
+            _lib.ptr(texcoords), _lib.ptr(colors), _lib.ptr(alphas), _lib.ptr(quads), hw[0], hw[1], channels,
+            colors.shape[0], float(tint_ratio), int(bool(modulate_alpha)), _lib.stream()))
+        return g_uv, g_colors, g_alphas, None, None, None, None, None
 
 
 class ImageTexture(StrokeTexture):
@@ -121,10 +122,7 @@ class ImageTexture(StrokeTexture):
         image = (image / 255.).permute(2, 0, 1).unsqueeze(0)  # [1, C, H, W]
         image = F.interpolate(image, tuple(texture_size), mode='area')
         self.register_buffer('image_texture', image, persistent=False)
-        c, h, w = image.shape[1:]
-        texels = torch.zeros(h, w, 4, dtype=torch.float32)
-        texels[..., :min(c, 4)] = image[0, :4].permute(1, 2, 0)
-        self.register_buffer('texels', texels, persistent=False)  # [H, W, 4] device layout
+        self._packed_key, self._packed_quads = None, None
 
     @property
     def channels(self):
@@ -134,13 +132,35 @@ class ImageTexture(StrokeTexture):
         if self.channels not in (1, 3):
             raise AssertionError('Texture should be RGB or grayscale')
 
+    @property
+    def size(self):
+        return tuple(int(v) for v in self.image_texture.shape[2:])
+
+    def packed(self):
+        """The image in the kernels' quad layout on the image's device (packed again
+        after the image buffer moves or changes)."""
+        img = self.image_texture
+        key = (img.device, img.data_ptr(), img._version)
+        if self._packed_key != key:
+            _lib.require_cuda('image_texture', img, contiguous=False)
+            (h, w), c = self.size, self.channels
+            lib = _lib.lib()
+            n = int(lib.n3ds_texture_pack_floats(h, w, c))
+            if n < 0:
+                _lib.check(1)
+            quads = torch.empty(n, dtype=torch.float32, device=img.device)
+            src = img[0].contiguous()
+            _lib.check(lib.n3ds_texture_pack(_lib.ptr(quads), _lib.ptr(src), h, w, c, _lib.stream()))
+            self._packed_key, self._packed_quads = key, quads
+        return self._packed_quads
+
     def sample_texture(self, texcoords):
         """textures.py:51-61: texture values [N, C] at texcoords [N, 2]."""
         self._check_channels()
         tc = texcoords.reshape(-1, 2)
         ones = torch.ones(tc.shape[0], 3, dtype=torch.float32, device=tc.device)
         alphas = torch.zeros(tc.shape[0], dtype=torch.float32, device=tc.device)
-        tint, _ = _texture_fn.apply(tc, ones, alphas, self.texels, self.channels, 1.0, False)
+        tint, _ = _texture_fn.apply(tc, ones, alphas, self.packed(), self.size, self.channels, 1.0, False)
         return tint[:, :self.channels]
 
     def forward(self, texcoords, colors, alphas):
@@ -149,6 +169,6 @@ class ImageTexture(StrokeTexture):
         pre_shape = colors.shape[:-1]
         modulate = bool(self.modulate_alpha)
         colors_out, alphas_out = _texture_fn.apply(texcoords.reshape(-1, 2), colors.reshape(-1, 3),
-                                                   alphas.reshape(-1), self.texels, self.channels,
+                                                   alphas.reshape(-1), self.packed(), self.size, self.channels,
                                                    float(self.tint_ratio), modulate)
         return colors_out.reshape(*pre_shape, 3), (alphas_out.reshape(*pre_shape) if modulate else alphas)

@@ -154,3 +154,24 @@ def test_texture_edge_cases(dev):
     rgba = ImageTexture(np.zeros((4, 4, 4), dtype=np.float32), (4, 4)).to(dev)
     with pytest.raises(AssertionError):
         rgba(tc.to(dev), col.to(dev), al.to(dev))
+
+
+@pytest.mark.parametrize('channels', [3, 1])
+def test_texture_pack_layout(channels, dev):
+    """Entry (j, i) of the packed image holds corners (i-1, j-1), (i, j-1), (i-1, j), (i, j)
+    channel by channel, zero off the image; the pack follows changes of the image buffer."""
+    tex = ImageTexture(_image(channels + 20, 7, 5, channels), (7, 5)).to(dev)
+    q = tex.packed().cpu().numpy()
+    img = tex.image_texture[0].cpu().numpy()
+    h, w = 7, 5
+    f = 16 if channels == 3 else 4
+    want = np.zeros(((h + 1) * (w + 1), f), dtype=np.float32)
+    for j in range(h + 1):
+        for i in range(w + 1):
+            for k, (dx, dy) in enumerate(((0, 0), (1, 0), (0, 1), (1, 1))):
+                x, y = i - 1 + dx, j - 1 + dy
+                if 0 <= x < w and 0 <= y < h:
+                    want[j * (w + 1) + i, k * channels:(k + 1) * channels] = img[:, y, x]
+    np.testing.assert_array_equal(q.reshape(-1, f), want)
+    tex.image_texture.mul_(0.5)
+    np.testing.assert_array_equal(tex.packed().cpu().numpy().reshape(-1, f), want * 0.5)

@@ -26,13 +26,16 @@ def test_image_setup_matches_reference_steps(tmp_path):
     want = F.interpolate((torch.from_numpy(arr.astype(np.float32)) / 255.).permute(2, 0, 1)[None], (20, 15),
                          mode='area')
     assert torch.equal(tex.image_texture, want)
-    assert tex.texels.shape == (20, 15, 4) and tex.channels == 3 and tex.tint_ratio == 0.5
-    assert torch.equal(tex.texels[..., :3], want[0].permute(1, 2, 0)) and not tex.texels[..., 3].any()
+    assert tex.size == (20, 15) and tex.channels == 3 and tex.tint_ratio == 0.5
+    lib = _lib.lib()
+    assert lib.n3ds_texture_pack_floats(20, 15, 3) == 21 * 16 * 16
+    assert lib.n3ds_texture_pack_floats(8, 8, 1) == 9 * 9 * 4
+    assert lib.n3ds_texture_pack_floats(8, 8, 4) == -1 and b'RGB or grayscale' in lib.n3ds_last_error()
+    assert lib.n3ds_texture_pack_floats(1 << 14, 1 << 13, 3) == -1
     # a grey image keeps one channel; an array gives the same buffers as the file
     grey = g.integers(0, 256, (8, 8)).astype(np.uint8)
     t1, t2 = ImageTexture(_png(tmp_path, grey), (8, 8)), ImageTexture(grey.astype(np.float32), (8, 8))
     assert t1.channels == 1 and torch.equal(t1.image_texture, t2.image_texture)
-    assert torch.equal(t1.texels, t2.texels)
     assert tex.modulate_alpha is False and ImageTexture(grey, (4, 4), modulate_alpha=True).modulate_alpha
 
 

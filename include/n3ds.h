@@ -343,6 +343,24 @@ int64_t n3ds_texture_pack_floats(int64_t height, int64_t width, int64_t channels
 int n3ds_texture_pack(float* quads, const float* image, int64_t height, int64_t width, int64_t channels,
                       void* stream);
 
+/* ------------------------------------------------------------------------
+ * Proposal-level interval resampling.
+ *
+ * Replaces the torch ops of source/utils/stepfun.py:sample_intervals
+ * (:280-317, with sample :216-254, invert_cdf :195-202, integrate_weights
+ * :149-170, sorted_interp :5-24), called at models.py:175-181.
+ * t [R, n_bins+1] sorted bin edges, w_logits [R, n_bins]; u_base [n_samples]
+ * the sample positions (the reference's torch.linspace); jitter [R,
+ * jitter_width] (torch.rand, jitter_width 1 = single_jitter, or n_samples)
+ * or NULL, scaled by max_jitter; t_out [R, n_samples+1].  Centres are the
+ * inverse CDF of softmax(w_logits) at u; fenceposts are their midpoints and
+ * the end reflections clamped to [minval, maxval].  n_samples > 1,
+ * n_bins + 1 <= 8192.
+ * ------------------------------------------------------------------------ */
+int n3ds_sample_intervals(float* t_out, const float* t, const float* w_logits, const float* u_base,
+                          const float* jitter, int jitter_width, float max_jitter, int64_t n_rays, int n_bins,
+                          int n_samples, float minval, float maxval, void* stream);
+
 /* Adjoint of n3ds_texture_forward: what torch's autograd gives the ops of
  * textures.py:51-82 (the image is a non-persistent buffer without gradient).
  * grad_colors [N,3] and, when modulate_alpha, grad_alphas [N] are written;

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libn3ds.so')
 BUILD_DIR = os.path.join(HERE, '_build')
 SOURCES = ['n3ds_capi.cpp', 'n3ds_strokes.hip', 'n3ds_compose.hip', 'n3ds_grid.hip', 'n3ds_render.hip', 'n3ds_optim.hip',
-           'n3ds_texture.hip']
+           'n3ds_texture.hip', 'n3ds_stepfun.hip']
 ARCH = os.environ.get('N3DS_OFFLOAD_ARCH', 'gfx950')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 

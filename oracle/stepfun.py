@@ -1,0 +1,66 @@
+"""Oracle restatement of the proposal-level interval sampler (test infrastructure only).
+
+Follows ``source/utils/stepfun.py``: ``sorted_interp`` (:5-24, the masked max / min
+interval search, restated literally), ``integrate_weights`` (:149-170),
+``invert_cdf`` (:195-202) and ``sample_intervals`` (:280-317) with the sample
+positions ``u`` given explicitly (the reference draws them in ``sample``,
+:216-254; ``sample_u`` restates that formula for given jitter draws).  numpy,
+float64.  Pinned by ``tests/test_oracle_stepfun.py`` against ``numpy.interp``
+(which the reference's own ``invert_cdf_np`` uses, :205-213) and closed forms.
+"""
+import numpy as np
+
+
+def softmax(x):
+    m = np.max(x, axis=-1, keepdims=True)
+    e = np.exp(x - m)
+    return e / e.sum(-1, keepdims=True)
+
+
+def integrate_weights(w):
+    """stepfun.py:149-170: [0, min(cumsum(w[:-1]), 1), 1]."""
+    cw = np.minimum(np.cumsum(w[..., :-1], axis=-1), 1.0)
+    shape = cw.shape[:-1] + (1,)
+    return np.concatenate([np.zeros(shape), cw, np.ones(shape)], -1)
+
+
+def sorted_interp(x, xp, fp):
+    """stepfun.py:5-24, literally: x [..., n], xp / fp [..., m+1]."""
+    mask = x[..., None, :] >= xp[..., :, None]  # [..., m+1, n]
+
+    def find_interval(v):
+        v0 = np.max(np.where(mask, v[..., None], v[..., :1, None]), -2)
+        v1 = np.min(np.where(~mask, v[..., None], v[..., -1:, None]), -2)
+        return v0, v1
+
+    fp0, fp1 = find_interval(fp)
+    xp0, xp1 = find_interval(xp)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        offset = np.clip(np.nan_to_num((x - xp0) / (xp1 - xp0), nan=0.0), 0, 1)
+    return fp0 + offset * (fp1 - fp0)
+
+
+def invert_cdf(u, t, w_logits):
+    """stepfun.py:195-202."""
+    return sorted_interp(u, integrate_weights(softmax(w_logits)), t)
+
+
+def sample_u(num_samples, rand_draws=None, eps=float(np.finfo(np.float32).eps)):
+    """stepfun.py:234-250 with deterministic_center=True: u [..., n] for jitter draws
+    [..., 1 or n] in [0, 1) (None: no jitter)."""
+    if rand_draws is None:
+        pad = 1 / (2 * num_samples)
+        return np.linspace(pad, 1. - pad - eps, num_samples)
+    u_max = eps + (1 - eps) / num_samples
+    max_jitter = (1 - u_max) / (num_samples - 1) - eps
+    return np.linspace(0, 1 - u_max, num_samples) + rand_draws * max_jitter
+
+
+def sample_intervals(u, t, w_logits, domain=(-np.inf, np.inf)):
+    """stepfun.py:280-317 for given sample positions u [..., n] (n > 1)."""
+    u = np.broadcast_to(u, t.shape[:-1] + (u.shape[-1],))
+    centers = invert_cdf(u, t, w_logits)
+    mid = (centers[..., 1:] + centers[..., :-1]) / 2
+    first = np.maximum(2 * centers[..., :1] - mid[..., :1], domain[0])
+    last = np.minimum(2 * centers[..., -1:] - mid[..., -1:], domain[1])
+    return np.concatenate([first, mid, last], -1)

@@ -314,6 +314,73 @@ def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
                                 'grad_alphas': r(g_al, og[1]), 'grad_texcoords': r(g_tc, og[2])}}
 
 
+def stepfun_block(seed=0, n_rays=65536, n_bins=64, n_samples=128, steps=20):
+    """Proposal-level resampling (stepfun.sample_intervals, single jitter) of 65536 rays from
+    64 weighted bins to 128 intervals: kernel time, GB/s against 8 TB/s, the reference's torch
+    formulation (masked max / min interval search) on the same GPU, and the error against
+    the float64 oracle on the first 256 rays."""
+    import numpy as np
+    import torch
+    from neural3dstrokes_amd import stepfun
+    from oracle import stepfun as osf
+    dev = torch.device('cuda:0')
+    gd = torch.Generator(device=dev).manual_seed(seed)
+    t = torch.sort(torch.rand(n_rays, n_bins + 1, device=dev, generator=gd), -1).values
+    t[:, 0], t[:, -1] = 0.0, 1.0
+    w = torch.rand(n_rays, n_bins, device=dev, generator=gd)
+    logits = stepfun.resample_logits(t, w)
+
+    def ours():
+        return stepfun.sample_intervals(True, t, logits, n_samples, single_jitter=True, domain=(0.0, 1.0))
+
+    def torch_ops():
+        # stepfun.py:5-24, 149-170, 195-317 in torch
+        eps = torch.finfo(torch.float32).eps
+        u_max = eps + (1 - eps) / n_samples
+        mj = (1 - u_max) / (n_samples - 1) - eps
+        u = torch.linspace(0, 1 - u_max, n_samples, device=dev) + torch.rand(n_rays, 1, device=dev) * mj
+        ww = torch.softmax(logits, -1)
+        cw = torch.cumsum(ww[..., :-1], -1).clamp_max(1)
+        cw0 = torch.cat([torch.zeros(n_rays, 1, device=dev), cw, torch.ones(n_rays, 1, device=dev)], -1)
+        mask = u[..., None, :] >= cw0[..., :, None]
+
+        def fi(x):
+            return (torch.max(torch.where(mask, x[..., None], x[..., :1, None]), -2).values,
+                    torch.min(torch.where(~mask, x[..., None], x[..., -1:, None]), -2).values)
+        fp0, fp1 = fi(t)
+        xp0, xp1 = fi(cw0)
+        c = fp0 + torch.clip(torch.nan_to_num((u - xp0) / (xp1 - xp0), 0), 0, 1) * (fp1 - fp0)
+        mid = (c[..., 1:] + c[..., :-1]) / 2
+        return torch.cat([(2 * c[..., :1] - mid[..., :1]).clamp_min(0), mid,
+                          (2 * c[..., -1:] - mid[..., -1:]).clamp_max(1)], -1)
+
+    times = {}
+    for name, fn, reps in (('kernel', ours, steps), ('torch_ops', torch_ops, 3)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / reps
+    torch.manual_seed(seed)
+    out = ours()
+    torch.manual_seed(seed)
+    draws = torch.rand(n_rays, 1, device=dev)[:256].double().cpu().numpy()
+    k = 256
+    want = osf.sample_intervals(osf.sample_u(n_samples, draws), t[:k].double().cpu().numpy(),
+                                logits[:k].double().cpu().numpy(), (0.0, 1.0))
+    got = out[:k].double().cpu().numpy()
+    nbytes = n_rays * 4 * ((n_bins + 1) + n_bins + (n_samples + 1) + 1)  # t, logits, fenceposts, jitter
+    return {'workload': f'{n_rays} rays, {n_bins} bins -> {n_samples} intervals, single jitter, domain [0, 1]',
+            'kernel_ms': times['kernel'], 'torch_ops_ms': times['torch_ops'],
+            'speedup_vs_torch_ops': times['torch_ops'] / times['kernel'], 'bytes': nbytes,
+            'GBps': nbytes / (times['kernel'] * 1e-3) / 1e9, 'frac_hbm': nbytes / (times['kernel'] * 1e-3) / 8e12,
+            'max_abs_err_vs_fp64': float(np.abs(got - want).max())}
+
+
 def memory_block(steps=3, warmup=1):
     """640,000 rays x 192 samples x 1000 strokes: fused fwd+bwd with the forward's hits / alpha
     kept for the adjoint versus full recomputation; throughput and peak HBM for both."""
@@ -501,6 +568,7 @@ def main():
     ap.add_argument('--fd-params', type=int, default=32, help='parameters for the finite-difference check (0: skip)')
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
     ap.add_argument('--texture', type=int, default=1, help='image-texture block (0: skip)')
+    ap.add_argument('--stepfun', type=int, default=1, help='proposal interval-resampling block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
     ap.add_argument('--autotune', type=int, default=1, help='tune launch shapes on the workload first (0: defaults)')
@@ -650,6 +718,8 @@ def main():
         out['hash_grid'] = grid_block()
     if args.texture:
         out['image_texture'] = texture_block()
+    if args.stepfun:
+        out['interval_resampling'] = stepfun_block()
     if args.memory:
         out['memory_bounded'] = memory_block()
     if args.multiscene:

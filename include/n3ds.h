@@ -361,6 +361,16 @@ int n3ds_sample_intervals(float* t_out, const float* t, const float* w_logits, c
                           const float* jitter, int jitter_width, float max_jitter, int64_t n_rays, int n_bins,
                           int n_samples, float minval, float maxval, void* stream);
 
+/* Replaces the torch ops of stepfun.py:max_dilate_weights (:136-146, over
+ * max_dilate :120-133, weight_to_pdf :109-112, pdf_to_weight :115-117),
+ * called at models.py:159-166.  t [R, n_bins+1] sorted, w [R, n_bins];
+ * t_out [R, 3 n_bins + 1] (t, t[:-1] - dilation, t[1:] + dilation in sorted
+ * order, clipped to [minval, maxval]); w_out [R, 3 n_bins] the max-dilated
+ * PDF back as weights, divided by max(sum, eps) when renormalize.
+ * n_bins <= 3276. */
+int n3ds_max_dilate_weights(float* t_out, float* w_out, const float* t, const float* w, int64_t n_rays, int n_bins,
+                            float dilation, float minval, float maxval, int renormalize, void* stream);
+
 /* Adjoint of n3ds_texture_forward: what torch's autograd gives the ops of
  * textures.py:51-82 (the image is a non-persistent buffer without gradient).
  * grad_colors [N,3] and, when modulate_alpha, grad_alphas [N] are written;

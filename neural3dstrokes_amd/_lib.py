@@ -57,6 +57,7 @@ SIGNATURES = {
                                           _I32, _F, _I32, _I32, _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P,
                                           _I32, _I32, _P, _I64, _P, _P, _P, _P]),
     'n3ds_sample_intervals': (_I32, [_P, _P, _P, _P, _P, _I32, _F, _I64, _I32, _I32, _F, _F, _P]),
+    'n3ds_max_dilate_weights': (_I32, [_P, _P, _P, _P, _I64, _I32, _F, _F, _F, _I32, _P]),
     'n3ds_texture_pack_floats': (_I64, [_I64, _I64, _I64]),
     'n3ds_texture_pack': (_I32, [_P, _P, _I64, _I64, _I64, _P]),
     'n3ds_texture_forward': (_I32, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _D, _I32, _P]),

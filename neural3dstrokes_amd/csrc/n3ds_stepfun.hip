@@ -118,6 +118,99 @@ __global__ __launch_bounds__(256) void sample_intervals_kernel(float* __restrict
     }
 }
 
+// number of entries of the sorted a[0..n) below x (strictly, or at most x when le)
+__device__ __forceinline__ int sf_rank(const float* a, int n, float x, bool le) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (le ? a[mid] <= x : a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// max_dilate_weights (stepfun.py:136-146 over max_dilate :120-133, weight_to_pdf :109-112,
+// pdf_to_weight :115-117): per ray, the edges t, t[:-1] - d and t[1:] + d merged in
+// sorted order (three sorted runs: each element's place is its index plus its rank in
+// the other two runs, ties broken run by run), clipped to the domain; the PDF at edge i
+// is the max of p_j over the bins with t_j - d <= e_i < t_{j+1} + d (a contiguous range
+// of j for sorted t, 0 when some bin does not cover it); back to weights, renormalised.
+// LDS per wave: t [m+1], p [m], merged edges [3m+1].
+__global__ __launch_bounds__(256) void max_dilate_kernel(float* __restrict__ t_out, float* __restrict__ w_out,
+                                                         const float* __restrict__ t, const float* __restrict__ w,
+                                                         int64_t n_rays, int m, float dilation, float minval,
+                                                         float maxval, int renormalize, int waves) {
+    extern __shared__ float sf_lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ne = 3 * m + 1;
+    float* ts = sf_lds + (int64_t)wave * (5 * m + 2);
+    float* ps = ts + (m + 1);
+    float* es = ps + m;
+    const float eps = 1.1920928955078125e-7f;  // float32 machine epsilon
+    for (int64_t r = (int64_t)blockIdx.x * waves + wave; r < n_rays; r += (int64_t)gridDim.x * waves) {
+        const float* tr = t + r * (m + 1);
+        const float* wr = w + r * m;
+        for (int j = lane; j <= m; j += 64) ts[j] = tr[j];
+        sf_wave_sync();
+        for (int j = lane; j < m; j += 64) ps[j] = wr[j] / fmaxf(__fsub_rn(ts[j + 1], ts[j]), eps);
+        // merged edge positions: run A = t [m+1], B = t[:-1] - d [m], C = t[1:] + d [m]
+        for (int k = lane; k < ne; k += 64) {
+            float v;
+            int pos;
+            if (k <= m) {  // A: after B / C entries strictly below
+                v = ts[k];
+                int lo = 0, hi = m;  // B below v: t_j - d < v
+                while (lo < hi) { const int mid = (lo + hi) >> 1; if (__fsub_rn(ts[mid], dilation) < v) lo = mid + 1; else hi = mid; }
+                int cb = lo;
+                lo = 0; hi = m;      // C below v: t_{j+1} + d < v
+                while (lo < hi) { const int mid = (lo + hi) >> 1; if (__fadd_rn(ts[mid + 1], dilation) < v) lo = mid + 1; else hi = mid; }
+                pos = k + cb + lo;
+            } else if (k <= 2 * m) {  // B: after A entries <= v and C entries < v
+                const int j = k - (m + 1);
+                v = __fsub_rn(ts[j], dilation);
+                const int ca = sf_rank(ts, m + 1, v, true);
+                int lo = 0, hi = m;
+                while (lo < hi) { const int mid = (lo + hi) >> 1; if (__fadd_rn(ts[mid + 1], dilation) < v) lo = mid + 1; else hi = mid; }
+                pos = j + ca + lo;
+            } else {  // C: after A and B entries <= v
+                const int j = k - (2 * m + 1);
+                v = __fadd_rn(ts[j + 1], dilation);
+                const int ca = sf_rank(ts, m + 1, v, true);
+                int lo = 0, hi = m;
+                while (lo < hi) { const int mid = (lo + hi) >> 1; if (__fsub_rn(ts[mid], dilation) <= v) lo = mid + 1; else hi = mid; }
+                pos = j + ca + lo;
+            }
+            es[pos] = fminf(fmaxf(v, minval), maxval);  // torch.clip(min, max)
+        }
+        sf_wave_sync();
+        float* tout = t_out + r * ne;
+        float* wout = w_out + r * (ne - 1);
+        float wsum = 0.f;
+        for (int i = lane; i < ne; i += 64) {
+            const float e = es[i];
+            tout[i] = e;
+            if (i == ne - 1) continue;
+            // bins covering e: t_j - d <= e (a prefix j < a) and t_{j+1} + d > e (a suffix j >= b)
+            int lo = 0, hi = m;
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if (__fsub_rn(ts[mid], dilation) <= e) lo = mid + 1; else hi = mid; }
+            const int a = lo;
+            lo = 0; hi = m;
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if (__fadd_rn(ts[mid + 1], dilation) <= e) lo = mid + 1; else hi = mid; }
+            const int b = lo;
+            float pd = (b == 0 && a == m) ? -INFINITY : 0.f;
+            for (int j = b; j < a; j++) pd = fmaxf(pd, ps[j]);
+            const float wd = __fmul_rn(pd, __fsub_rn(es[i + 1], e));
+            wout[i] = wd;
+            wsum += wd;
+        }
+        if (renormalize) {
+            wsum = fmaxf(wave_sum(wsum), eps);
+            for (int i = lane; i < ne - 1; i += 64) wout[i] = wout[i] / wsum;
+        }
+        sf_wave_sync();  // the next ray overwrites the slice
+    }
+}
+
 }  // namespace n3ds
 
 using namespace n3ds;
@@ -139,6 +232,23 @@ N3DS_EXPORT int n3ds_sample_intervals(float* t_out, const float* t, const float*
     hipLaunchKernelGGL(sample_intervals_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, (hipStream_t)stream,
                        t_out, t, w_logits, u_base, jitter, jitter_width, max_jitter, n_rays, n_bins, n_samples,
                        minval, maxval, waves);
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int n3ds_max_dilate_weights(float* t_out, float* w_out, const float* t, const float* w, int64_t n_rays,
+                                        int n_bins, float dilation, float minval, float maxval, int renormalize,
+                                        void* stream) {
+    N3DS_REQUIRE(n_bins >= 1 && 5 * (int64_t)n_bins + 2 <= 16384, "n_bins %d outside [1, 3276]", n_bins);
+    N3DS_REQUIRE(n_rays >= 0, "negative size");
+    if (n_rays == 0) return N3DS_OK;
+    N3DS_REQUIRE(t_out && w_out && t && w, "null argument");
+    const int per_wave = 5 * n_bins + 2;
+    const int waves = per_wave <= 4096 ? 4 : 1;
+    const size_t lds = (size_t)waves * per_wave * sizeof(float);
+    const int64_t blocks = std::min<int64_t>((n_rays + waves - 1) / waves, 256 * 32);
+    hipLaunchKernelGGL(max_dilate_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, (hipStream_t)stream, t_out,
+                       w_out, t, w, n_rays, n_bins, dilation, minval, maxval, renormalize, waves);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;
 }

@@ -1,7 +1,8 @@
 """Oracle restatement of the proposal-level interval sampler (test infrastructure only).
 
 Follows ``source/utils/stepfun.py``: ``sorted_interp`` (:5-24, the masked max / min
-interval search, restated literally), ``integrate_weights`` (:149-170),
+interval search, restated literally), ``max_dilate_weights`` (:109-146, sort and
+masked max restated literally), ``integrate_weights`` (:149-170),
 ``invert_cdf`` (:195-202) and ``sample_intervals`` (:280-317) with the sample
 positions ``u`` given explicitly (the reference draws them in ``sample``,
 :216-254; ``sample_u`` restates that formula for given jitter draws).  numpy,
@@ -64,3 +65,30 @@ def sample_intervals(u, t, w_logits, domain=(-np.inf, np.inf)):
     first = np.maximum(2 * centers[..., :1] - mid[..., :1], domain[0])
     last = np.minimum(2 * centers[..., -1:] - mid[..., -1:], domain[1])
     return np.concatenate([first, mid, last], -1)
+
+
+F32_EPS = float(np.finfo(np.float32).eps)
+
+
+def max_dilate(t, p, dilation, domain=(-np.inf, np.inf), edge_dtype=None):
+    """stepfun.py:120-133.  edge_dtype=np.float32 rounds the dilated edges t -+ d as the
+    reference's float32 tensors do (the sort keys; everything else stays float64)."""
+    t0 = t[..., :-1] - dilation
+    t1 = t[..., 1:] + dilation
+    if edge_dtype is not None:
+        t0 = t0.astype(edge_dtype).astype(np.float64)
+        t1 = t1.astype(edge_dtype).astype(np.float64)
+    td = np.clip(np.sort(np.concatenate([t, t0, t1], -1), -1), *domain)
+    cover = (t0[..., None, :] <= td[..., None]) & (t1[..., None, :] > td[..., None])
+    pd = np.max(np.where(cover, p[..., None, :], 0.0), -1)[..., :-1]
+    return td, pd
+
+
+def max_dilate_weights(t, w, dilation, domain=(-np.inf, np.inf), renormalize=False, eps=F32_EPS, edge_dtype=None):
+    """stepfun.py:136-146 with weight_to_pdf / pdf_to_weight (:109-117)."""
+    p = w / np.maximum(t[..., 1:] - t[..., :-1], eps)
+    td, pd = max_dilate(t, p, dilation, domain, edge_dtype)
+    wd = pd * (td[..., 1:] - td[..., :-1])
+    if renormalize:
+        wd = wd / np.maximum(wd.sum(-1, keepdims=True), eps)
+    return td, wd

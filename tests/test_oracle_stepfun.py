@@ -64,3 +64,34 @@ def test_outputs_sorted_and_zero_width_bins_get_no_centres():
     u = O.sample_u(40, np.random.default_rng(2).uniform(0, 1, (20, 1)))
     s = O.sample_intervals(u, t, logits, domain=(0.0, 1.0))
     assert np.all(np.diff(s, axis=-1) >= 0) and s.min() >= 0 and s.max() <= 1
+
+
+def test_max_dilate_closed_forms():
+    # no dilation keeps the step function: every weight survives on its own bin
+    t, _ = _rays(4, 5, 12, zero_bins=False)
+    w = np.random.default_rng(5).uniform(0.1, 1, (5, 12))
+    w /= w.sum(-1, keepdims=True)
+    td, wd = O.max_dilate_weights(t, w, 0.0, renormalize=True)
+    assert td.shape == (5, 37) and wd.shape == (5, 36)
+    np.testing.assert_allclose(wd.sum(-1), 1.0)
+    for r in range(5):
+        keep = np.diff(td[r]) > 0
+        np.testing.assert_allclose(wd[r][keep], w[r], rtol=1e-12)
+    # one bin [0, 1] dilated by 0.25 inside the domain [0, 1]: all mass stays on [0, 1]
+    td, wd = O.max_dilate_weights(np.array([[0.0, 1.0]]), np.array([[1.0]]), 0.25, domain=(0.0, 1.0),
+                                  renormalize=True)
+    np.testing.assert_array_equal(td, [[0.0, 0.0, 1.0, 1.0]])
+    np.testing.assert_array_equal(wd, [[0.0, 1.0, 0.0]])
+    # max dilation: the dilated pdf dominates the original pdf at every original edge
+    t, _ = _rays(6, 3, 20, zero_bins=False)
+    w = np.random.default_rng(7).uniform(0, 1, (3, 20))
+    p = w / np.diff(t, axis=-1)
+    td, wd = O.max_dilate_weights(t, w, 0.03)
+    pd = wd / np.maximum(np.diff(td, axis=-1), 1e-30)
+    for r in range(3):
+        for j in range(20):
+            mid = 0.5 * (t[r, j] + t[r, j + 1])
+            i = np.searchsorted(td[r], mid, side='right') - 1
+            assert pd[r, i] >= p[r, j] * (1 - 1e-12)
+    # total mass never shrinks without renormalisation
+    assert np.all(wd.sum(-1) >= w.sum(-1) * (1 - 1e-12))

@@ -113,8 +113,10 @@ class ImageTexture(StrokeTexture):
         super().__init__()
         for k, v in kwargs.items():
             setattr(self, k, v)
-        if isinstance(tint_image_path, (np.ndarray, torch.Tensor)):
-            image = torch.as_tensor(np.asarray(tint_image_path, dtype=np.float32))
+        if isinstance(tint_image_path, torch.Tensor):
+            image = tint_image_path.detach().to('cpu', torch.float32)
+        elif isinstance(tint_image_path, np.ndarray):
+            image = torch.from_numpy(np.asarray(tint_image_path, dtype=np.float32))
         else:
             image = torch.from_numpy(_load_image(tint_image_path))
         if image.ndim == 2:

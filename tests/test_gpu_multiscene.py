@@ -105,7 +105,7 @@ def test_multiscene_graph_step_replays_eager_steps():
     torch.cuda.synchronize()
     assert float(graph_t.opt.state[graph_t.shape]['step']) == 6
     for a, b in ((graph_t.shape, eager_t.shape), (graph_t.color, eager_t.color), (graph_t.density, eager_t.density)):
-        d = (a - b).abs()
+        d = (a - b).detach().abs()
         # Adam moves a parameter by at most ~lr per step whatever its gradient; a gradient that
         # is zero up to rounding may take either sign, so bound the rare outliers by that
         assert float(d.max()) <= 6 * 2e-3

@@ -36,6 +36,8 @@ def test_image_setup_matches_reference_steps(tmp_path):
     grey = g.integers(0, 256, (8, 8)).astype(np.uint8)
     t1, t2 = ImageTexture(_png(tmp_path, grey), (8, 8)), ImageTexture(grey.astype(np.float32), (8, 8))
     assert t1.channels == 1 and torch.equal(t1.image_texture, t2.image_texture)
+    t3 = ImageTexture(torch.from_numpy(grey.astype(np.float32)), (8, 8))
+    assert torch.equal(t3.image_texture, t2.image_texture)
     assert tex.modulate_alpha is False and ImageTexture(grey, (4, 4), modulate_alpha=True).modulate_alpha
 
 

@@ -64,3 +64,23 @@ def test_texture_rejects_cpu_tensors_and_bad_arguments():
                                      None) == 1
     assert lib.n3ds_texture_forward(None, None, None, None, None, ctypes.c_void_p(8), 4, 4, 3, 5, 1.0, 0,
                                     None) == 1
+
+
+def test_resampling_entry_points_validate_arguments():
+    """n3ds_sample_intervals / n3ds_max_dilate_weights reject bad sizes before any device work."""
+    lib = _lib.lib()
+    assert lib.n3ds_sample_intervals(None, None, None, None, None, 0, 0.0, 4, 8, 1, 0.0, 1.0, None) == 1
+    assert b'num_samples must be > 1' in lib.n3ds_last_error()
+    assert lib.n3ds_sample_intervals(None, None, None, None, None, 0, 0.0, 4, 8192, 4, 0.0, 1.0, None) == 1
+    assert lib.n3ds_sample_intervals(None, None, None, None, ctypes.c_void_p(16), 3, 0.0, 4, 8, 4, 0.0, 1.0,
+                                     None) == 1
+    assert b'jitter width' in lib.n3ds_last_error()
+    assert lib.n3ds_sample_intervals(None, None, None, None, None, 0, 0.0, 0, 8, 4, 0.0, 1.0, None) == 0
+    assert lib.n3ds_max_dilate_weights(None, None, None, None, 4, 0, 0.1, 0.0, 1.0, 1, None) == 1
+    assert lib.n3ds_max_dilate_weights(None, None, None, None, 4, 4000, 0.1, 0.0, 1.0, 1, None) == 1
+    assert lib.n3ds_max_dilate_weights(None, None, None, None, 0, 8, 0.1, 0.0, 1.0, 1, None) == 0
+    from neural3dstrokes_amd import stepfun
+    with pytest.raises(ValueError):
+        stepfun.sample_intervals(False, torch.zeros(2, 5), torch.zeros(2, 4), 1)
+    with pytest.raises(RuntimeError):
+        stepfun.max_dilate_weights(torch.zeros(2, 5), torch.zeros(2, 5), 0.1)

@@ -374,7 +374,46 @@ def stepfun_block(seed=0, n_rays=65536, n_bins=64, n_samples=128, steps=20):
                                 logits[:k].double().cpu().numpy(), (0.0, 1.0))
     got = out[:k].double().cpu().numpy()
     nbytes = n_rays * 4 * ((n_bins + 1) + n_bins + (n_samples + 1) + 1)  # t, logits, fenceposts, jitter
-    return {'workload': f'{n_rays} rays, {n_bins} bins -> {n_samples} intervals, single jitter, domain [0, 1]',
+
+    # weight dilation before resampling (models.py:159-166), dilation 0.005, renormalised
+    wn = w / w.sum(-1, keepdim=True)
+
+    def dilate():
+        return stepfun.max_dilate_weights(t, wn, 0.005, domain=(0.0, 1.0), renormalize=True)
+
+    def dilate_torch():
+        # stepfun.py:109-146 in torch
+        eps = torch.finfo(torch.float32).eps
+        p = wn / (t[..., 1:] - t[..., :-1]).clamp_min(eps)
+        t0, t1 = t[..., :-1] - 0.005, t[..., 1:] + 0.005
+        td = torch.clip(torch.sort(torch.cat([t, t0, t1], -1), -1).values, 0.0, 1.0)
+        pd = torch.max(torch.where((t0[..., None, :] <= td[..., None]) & (t1[..., None, :] > td[..., None]),
+                                   p[..., None, :], torch.zeros_like(p[..., None, :])), -1).values[..., :-1]
+        wd = pd * (td[..., 1:] - td[..., :-1])
+        return td, wd / wd.sum(-1, keepdim=True).clamp_min(eps)
+
+    for name, fn, reps in (('dilate', dilate, steps), ('dilate_torch', dilate_torch, 3)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / reps
+    td, wd = dilate()
+    otd, owd = osf.max_dilate_weights(t[:k].double().cpu().numpy(), wn[:k].double().cpu().numpy(),
+                                      float(np.float32(0.005)), (0.0, 1.0), True, edge_dtype=np.float32)
+    dbytes = n_rays * 4 * ((n_bins + 1) + n_bins + (3 * n_bins + 1) + 3 * n_bins)
+    dilation = {'kernel_ms': times['dilate'], 'torch_ops_ms': times['dilate_torch'],
+                'speedup_vs_torch_ops': times['dilate_torch'] / times['dilate'], 'bytes': dbytes,
+                'GBps': dbytes / (times['dilate'] * 1e-3) / 1e9,
+                'max_abs_err_vs_fp64': {'t': float(np.abs(td[:k].double().cpu().numpy() - otd).max()),
+                                        'w': float(np.abs(wd[:k].double().cpu().numpy() - owd).max())}}
+    return {'workload': f'{n_rays} rays, {n_bins} bins -> {n_samples} intervals, single jitter, domain [0, 1]; '
+                        f'dilation of the {n_bins} normalised weights by 0.005 (renormalised)',
+            'max_dilate_weights': dilation,
             'kernel_ms': times['kernel'], 'torch_ops_ms': times['torch_ops'],
             'speedup_vs_torch_ops': times['torch_ops'] / times['kernel'], 'bytes': nbytes,
             'GBps': nbytes / (times['kernel'] * 1e-3) / 1e9, 'frac_hbm': nbytes / (times['kernel'] * 1e-3) / 8e12,

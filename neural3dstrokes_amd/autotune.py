@@ -20,8 +20,8 @@ import torch
 from . import _lib
 
 PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3}
-SPACE = {'flush_pairs': (64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16),
-         'cull_group': (1, 2, 4, 8)}
+SPACE = {'flush_pairs': (32, 48, 64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16),
+         'cull_group': (1, 2, 4, 8, 12, 16)}
 _cache = {}
 
 

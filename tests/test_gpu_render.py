@@ -175,12 +175,20 @@ def test_render_results_do_not_depend_on_tuning():
         autotune.set('bwd_oversub', 4)
         autotune.set('cull_group', 4)
         (rgb1, acc1, d1), g1, _ = _run(sc, dev)
+        autotune.set('flush_pairs', 32)
+        autotune.set('bwd_oversub', 2)
+        autotune.set('cull_group', 16)
+        (rgb2, acc2, d2), g2, _ = _run(sc, dev)
+        autotune.set('flush_pairs', 16)
+        autotune.set('cull_group', 12)
+        (rgb3, acc3, d3), g3, _ = _run(sc, dev)
     finally:
         for k, v in old.items():
             autotune.set(k, v)
-    assert torch.equal(rgb0, rgb1) and torch.equal(acc0, acc1) and torch.equal(d0, d1)
-    for a, b in zip(g0, g1):
-        assert rel_err(a, b) < 1e-5
+    for rgb, acc, d, g in ((rgb1, acc1, d1, g1), (rgb2, acc2, d2, g2), (rgb3, acc3, d3, g3)):
+        assert torch.equal(rgb0, rgb) and torch.equal(acc0, acc) and torch.equal(d0, d)
+        for a, b in zip(g0, g):
+            assert rel_err(a, b) < 1e-5
 
 
 @pytest.mark.gpu

@@ -355,7 +355,7 @@ int n3ds_texture_pack(float* quads, const float* image, int64_t height, int64_t 
  * or NULL, scaled by max_jitter; t_out [R, n_samples+1].  Centres are the
  * inverse CDF of softmax(w_logits) at u; fenceposts are their midpoints and
  * the end reflections clamped to [minval, maxval].  n_samples > 1,
- * n_bins + 1 <= 8192.
+ * 2 (n_bins + 1) + n_samples <= 16384.
  * ------------------------------------------------------------------------ */
 int n3ds_sample_intervals(float* t_out, const float* t, const float* w_logits, const float* u_base,
                           const float* jitter, int jitter_width, float max_jitter, int64_t n_rays, int n_bins,

@@ -3,12 +3,12 @@
 // sample (:216-254), invert_cdf (:195-202), integrate_weights (:149-170) and
 // sorted_interp (:5-24), as called at models.py:168-181.
 //
-// Mapping: one wavefront per ray.  The ray's bin edges t [m+1] and CDF
-// cw0 [m+1] are staged in the wave's LDS slice: softmax of the logits by two
+// Mapping: one wavefront per ray.  The ray's bin edges t [m+1], CDF
+// cw0 [m+1] and centres [n] are staged in the wave's LDS slice: softmax of the logits by two
 // wave reductions (max, sum of exp), the CDF by a 64-lane inclusive scan with
 // a running carry, clamped to 1 and closed with exact 0 and 1.  Each lane then
-// owns output fenceposts: the inverse CDF at a sample u is an upper-bound
-// binary search in cw0 (cw0 is non-decreasing, so "last entry <= u" is what
+// evaluates centres: the inverse CDF at a sample u is an upper-bound binary
+// search in cw0 (cw0 is non-decreasing, so "last entry <= u" is what
 // sorted_interp's masked max / min select for sorted t) and one linear
 // interpolation; fencepost o takes the midpoint of centres o-1 and o, and the
 // first / last the reflection of the end midpoint clamped to the domain.
@@ -21,7 +21,7 @@
 
 namespace n3ds {
 
-constexpr int SF_MAX_EDGES = 8192;  // m + 1 per ray (LDS: 8 B per edge, <= 64 KB per block)
+constexpr int SF_MAX_FLOATS = 16384;  // LDS floats per ray slice: 2 (m + 1) + n (<= 64 KB per block)
 
 // LDS ordering between the lanes of one wavefront (no block barrier)
 __device__ __forceinline__ void sf_wave_sync() {
@@ -61,8 +61,9 @@ __global__ __launch_bounds__(256) void sample_intervals_kernel(float* __restrict
                                                                float minval, float maxval, int waves) {
     extern __shared__ float sf_lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    float* cw = sf_lds + (int64_t)wave * 2 * (m + 1);
+    float* cw = sf_lds + (int64_t)wave * (2 * (m + 1) + n);
     float* te = cw + (m + 1);
+    float* cs = te + (m + 1);
     for (int64_t r = (int64_t)blockIdx.x * waves + wave; r < n_rays; r += (int64_t)gridDim.x * waves) {
         const float* tr = t + r * (m + 1);
         const float* lr = logits + r * m;
@@ -95,22 +96,23 @@ __global__ __launch_bounds__(256) void sample_intervals_kernel(float* __restrict
         sf_wave_sync();
         float* outr = t_out + r * (n + 1);
         const float* jr = jitter ? jitter + r * jitter_width : nullptr;
-        auto center = [&](int i) {
+        for (int i = lane; i < n; i += 64) {
             float u = u_base[i];
             if (jr) u = __fadd_rn(u, __fmul_rn(jr[jitter_width == 1 ? 0 : i], max_jitter));
-            return sf_center(cw, te, m, u);
-        };
+            cs[i] = sf_center(cw, te, m, u);
+        }
+        sf_wave_sync();
         for (int o = lane; o <= n; o += 64) {
             float val;
             if (o == 0 || o == n) {
                 const int e = o == 0 ? 0 : n - 1, nb = o == 0 ? 1 : n - 2;  // end centre and its neighbour
-                const float ce = center(e), cn = center(nb);
+                const float ce = cs[e], cn = cs[nb];
                 const float midv = (o == 0 ? __fadd_rn(cn, ce) : __fadd_rn(ce, cn)) / 2.f;
                 const float refl = __fsub_rn(2.f * ce, midv);
                 // clamp_min / clamp_max keep NaN
                 val = o == 0 ? (refl < minval ? minval : refl) : (refl > maxval ? maxval : refl);
             } else {
-                val = __fadd_rn(center(o), center(o - 1)) / 2.f;
+                val = __fadd_rn(cs[o], cs[o - 1]) / 2.f;
             }
             outr[o] = val;
         }
@@ -219,15 +221,16 @@ N3DS_EXPORT int n3ds_sample_intervals(float* t_out, const float* t, const float*
                                       const float* jitter, int jitter_width, float max_jitter, int64_t n_rays,
                                       int n_bins, int n_samples, float minval, float maxval, void* stream) {
     N3DS_REQUIRE(n_samples > 1, "num_samples must be > 1, is %d.", n_samples);
-    N3DS_REQUIRE(n_bins >= 1 && n_bins + 1 <= SF_MAX_EDGES, "n_bins %d outside [1, %d]", n_bins, SF_MAX_EDGES - 1);
+    N3DS_REQUIRE(n_bins >= 1 && 2 * ((int64_t)n_bins + 1) + n_samples <= SF_MAX_FLOATS,
+                 "2 (n_bins + 1) + n_samples = %lld exceeds %d", 2 * ((long long)n_bins + 1) + n_samples, SF_MAX_FLOATS);
     N3DS_REQUIRE(n_rays >= 0, "negative size");
     N3DS_REQUIRE(!jitter || jitter_width == 1 || jitter_width == n_samples,
                  "jitter width must be 1 or n_samples (got %d)", jitter_width);
     if (n_rays == 0) return N3DS_OK;
     N3DS_REQUIRE(t_out && t && w_logits && u_base, "null argument");
-    const int edges = n_bins + 1;
-    const int waves = edges <= 2048 ? 4 : 1;
-    const size_t lds = (size_t)waves * 2 * edges * sizeof(float);
+    const int per_wave = 2 * (n_bins + 1) + n_samples;
+    const int waves = per_wave <= SF_MAX_FLOATS / 4 ? 4 : 1;
+    const size_t lds = (size_t)waves * per_wave * sizeof(float);
     const int64_t blocks = std::min<int64_t>((n_rays + waves - 1) / waves, 256 * 32);
     hipLaunchKernelGGL(sample_intervals_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, (hipStream_t)stream,
                        t_out, t, w_logits, u_base, jitter, jitter_width, max_jitter, n_rays, n_bins, n_samples,

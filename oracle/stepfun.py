@@ -13,9 +13,11 @@ import numpy as np
 
 
 def softmax(x):
+    """torch.softmax semantics: a row of all -inf gives NaN."""
     m = np.max(x, axis=-1, keepdims=True)
-    e = np.exp(x - m)
-    return e / e.sum(-1, keepdims=True)
+    with np.errstate(invalid='ignore'):
+        e = np.exp(x - m)
+        return e / e.sum(-1, keepdims=True)
 
 
 def integrate_weights(w):

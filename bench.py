@@ -274,6 +274,12 @@ def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
                                              _lib.ptr(ga), _lib.ptr(tc), _lib.ptr(col), _lib.ptr(al), _lib.ptr(T), 256,
                                              256, 3, n_pairs, 0.7, 1, _lib.stream()))
 
+    def bwd_chain():
+        # inside the stroke -> texture -> compose chain the texcoords carry no gradient
+        _lib.check(lib.n3ds_texture_backward(_lib.ptr(g_col), _lib.ptr(g_al), None, _lib.ptr(gc), _lib.ptr(ga),
+                                             _lib.ptr(tc), _lib.ptr(col), _lib.ptr(al), _lib.ptr(T), 256, 256, 3,
+                                             n_pairs, 0.7, 1, _lib.stream()))
+
     def torch_ops():
         t, c, a = (x.detach().requires_grad_(True) for x in (tc, col, al))
         tint = F.grid_sample(tex.image_texture, t.reshape(1, 1, -1, 2) * 2 - 1, align_corners=True)[0, :, 0, :].T
@@ -282,7 +288,8 @@ def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
         torch.autograd.backward([cc, aa], [gc, ga])
 
     times = {}
-    for name, fn, reps in (('fwd', fwd, steps), ('bwd', bwd, steps), ('torch_fwd_bwd', torch_ops, 2)):
+    for name, fn, reps in (('fwd', fwd, steps), ('bwd_chain', bwd_chain, steps), ('bwd', bwd, steps),
+                           ('torch_fwd_bwd', torch_ops, 2)):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -306,6 +313,8 @@ def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
     return {'workload': f'{n_pairs} (sample, stroke) pairs = 2048 rays x 128 samples x 500 strokes, 256x256 RGB '
                         'texture, tint_ratio 0.7, alpha modulated, texcoord gradient on',
             'fwd_ms': times['fwd'], 'bwd_ms': times['bwd'], 'torch_ops_fwd_bwd_ms': times['torch_fwd_bwd'],
+            'bwd_ms_in_stroke_chain': times['bwd_chain'],
+            'bwd_in_stroke_chain_GBps': n_pairs * 40 / (times['bwd_chain'] * 1e-3) / 1e9,
             'speedup_vs_torch_ops': times['torch_fwd_bwd'] / (times['fwd'] + times['bwd']),
             'fwd_bytes': b_fwd, 'bwd_bytes': b_bwd,
             'fwd_GBps': b_fwd / (times['fwd'] * 1e-3) / 1e9, 'bwd_GBps': b_bwd / (times['bwd'] * 1e-3) / 1e9,

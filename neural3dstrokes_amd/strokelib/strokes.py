@@ -153,9 +153,14 @@ class _stroke_fn(Function):
             ctx.save_for_backward(x, radius, viewdir, alpha, shape_params, color_params)
             ctx.cfg = (sdf_id, color_id, sdf_delta, use_laplace_transform, inv_scale_radius, pre_shape,
                        get_adjoint())
+        texo = None
+        if return_texcoord:
+            # the reference's backward discards grad_texcoord (strokes.py:188-224): no gradient
+            # flows through texture coordinates, so consumers need not compute one
+            texo = tex.reshape(*pre_shape, n_str, 2)
+            ctx.mark_non_differentiable(texo)
         return (alpha.reshape(*pre_shape, n_str), color.reshape(*pre_shape, n_str, -1),
-                None if no_sdf else sdf.reshape(*pre_shape, n_str),
-                tex.reshape(*pre_shape, n_str, 2) if return_texcoord else None)
+                None if no_sdf else sdf.reshape(*pre_shape, n_str), texo)
 
     @staticmethod
     @once_differentiable

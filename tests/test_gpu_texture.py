@@ -120,7 +120,7 @@ def test_texture_in_stroke_chain(dev):
     for use_kernel in (True, False):
         leaves = [t.to(dev).requires_grad_(True) for t in (sp, cp, dp)]
         a, c, _, t = fn(x.to(dev), r.to(dev), vd.to(dev), leaves[0], leaves[1], 1.0, False, False, True, True)
-        assert t.shape == (64, 32, 8, 2)
+        assert t.shape == (64, 32, 8, 2) and not t.requires_grad  # no gradient path, as in the reference
         if use_kernel:
             c, a = tex(t, c, a)
         else:

@@ -113,6 +113,13 @@ __global__ __launch_bounds__(TEX_THREADS) void texture_pack_kernel(float* __rest
     }
 }
 
+// the forward streams its per-pair data non-temporally, so it does not displace the
+// packed texture from L2 (1.99 -> 1.87 ms on 131M pairs; the same in the backward
+// measured slower)
+__device__ __forceinline__ float2 ntload2(const float* p, int64_t i) {
+    return make_float2(__builtin_nontemporal_load(p + 2 * i), __builtin_nontemporal_load(p + 2 * i + 1));
+}
+
 template <int C>
 __device__ __forceinline__ float tint_mean(const TexSample& s) {
     return C == 3 ? (s.t[0] + s.t[1] + s.t[2]) / 3.f : s.t[0];
@@ -128,14 +135,15 @@ __global__ __launch_bounds__(TEX_THREADS) void texture_fwd_kernel(float* __restr
                                                                   int64_t n, float tr, float one_minus_tr) {
     const int64_t stride = (int64_t)gridDim.x * TEX_THREADS;
     for (int64_t i = (int64_t)blockIdx.x * TEX_THREADS + threadIdx.x; i < n; i += stride) {
-        const float2 uv = reinterpret_cast<const float2*>(texcoords)[i];
+        const float2 uv = ntload2(texcoords, i);
         const TexSample s = tex_sample<C, false>(quads, H, W, uv.x, uv.y);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float m = __fadd_rn(__fmul_rn(colors[i * 3 + c], tr), one_minus_tr);
-            colors_out[i * 3 + c] = __fmul_rn(m, s.t[C == 3 ? c : 0]);
+            const float m = __fadd_rn(__fmul_rn(__builtin_nontemporal_load(colors + i * 3 + c), tr), one_minus_tr);
+            __builtin_nontemporal_store(__fmul_rn(m, s.t[C == 3 ? c : 0]), colors_out + i * 3 + c);
         }
-        if (MOD) alphas_out[i] = __fmul_rn(alphas[i], tint_mean<C>(s));
+        if (MOD) __builtin_nontemporal_store(__fmul_rn(__builtin_nontemporal_load(alphas + i), tint_mean<C>(s)),
+                                             alphas_out + i);
     }
 }
 

@@ -124,18 +124,39 @@ __device__ __forceinline__ float eval_tetrahedron(V3 p) {
 
 // ---------------------------------------------------------------------------
 // quadratic Bezier tube (closed-form cubic solve), strokes_sdf.h:647-705
+//
+// The closed form picks t from the depressed cubic s^3 + 3 p s + q = 0
+// (s = t + kx), whose coefficients are differences of nearly equal terms.
+// The set-up (kx, ky, kz, p, q, h) is therefore computed in double (a few
+// dozen FP64 operations, the unpacked FP32 rate on CDNA4), the transcendentals
+// in single precision.  With its 1e-8 regularisers the closed form does not
+// return a root of that cubic but the exact root of a nearby one:
+//   one-root branch:  s^3 + 3 p' s + q = 0,  p' = cbrt(p^3 + 2.5e-9)
+//     (the two cube roots u, w satisfy u^3 + w^3 = -q, u w = -p');
+//   three-root branch, second candidate:  s^3 + 3 (p - 1e-8) s + q'' = 0,
+//     q'' = q (p - 1e-8) / (p + 5e-9 / z),  z = sqrt(1e-8 - p)
+//     (s = 2 z cos(v + 2 pi / 3) with cos 3v = q / (2 p z + 1e-8)).
+// For those candidates t is refined by one Newton step on that cubic, written
+// as F(t) = g(t)/|b|^2 + c1 (t + kx) + c0 with the geometric residual
+// g(t) = (P(t) - pos) . P'(t)/2 (no cancellation) and the small regulariser
+// terms c1, c0 in closed form; the exact adjoint differentiates F = 0
+// implicitly.  The three-root branch's first candidate (m + n) z - kx is not a
+// root of any of these cubics and keeps the closed form and its derivative.
 // ---------------------------------------------------------------------------
 struct QuadSolve {
     V3 a, b, c, d;
-    float kx, ky, kz, p, q, h;
-    // one-root branch
-    float x0, x1;
-    // three-root branch
-    float z, v, m, n;
-    bool one;      // h >= 0
-    bool use_b;    // three-root branch picked the second candidate
-    float t;       // chosen parameter (clamped)
-    float dist2;   // |P(t) - pos|^2
+    float kk, kx, ky, kz, p, q;
+    float z, v, m, n;   // three-root branch
+    float den, sq;      // acos argument's denominator, sqrt(1 - arg^2)
+    float c1, c0;       // regulariser terms of F
+    float ep, eq;       // dC/dp (one-root: per unit of s), dC/dq of C = c1 (t + kx) + c0
+    bool one;           // h >= 0
+    bool use_b;         // three-root branch picked the second candidate
+    bool clamped;       // acos argument outside (-1, 1)
+    bool root;          // t is an interior root of F (refined; implicit adjoint)
+    float traw;         // unclamped parameter of the pick (outside (0, 1) when clamped)
+    float t;            // chosen parameter (clamped)
+    float dist2;        // |P(t) - pos|^2
 };
 
 __device__ __forceinline__ float cbrt_signed(float x) { return copysignf(powf(fabsf(x), 1.0f / 3.0f), x); }
@@ -147,42 +168,91 @@ __device__ __forceinline__ float quad_d2(V3 d, V3 c, V3 b, float t) {
 
 __device__ __forceinline__ QuadSolve quad_solve(V3 pos, const float* prm) {
     QuadSolve Q;
-    V3 A = ld3(prm), B = ld3(prm + 3), C = ld3(prm + 6);
-    Q.a = B - A;
-    Q.b = A - 2.f * B + C;
+    const double Ax = prm[0], Ay = prm[1], Az = prm[2];
+    const double Bx = prm[3], By = prm[4], Bz = prm[5];
+    const double Cx = prm[6], Cy = prm[7], Cz = prm[8];
+    const double ax = Bx - Ax, ay = By - Ay, az = Bz - Az;
+    const double bx = Ax - 2.0 * Bx + Cx, by = Ay - 2.0 * By + Cy, bz = Az - 2.0 * Bz + Cz;
+    const double dx = Ax - (double)pos.x, dy = Ay - (double)pos.y, dz = Az - (double)pos.z;
+    const double kk = 1.0 / (bx * bx + by * by + bz * bz);
+    const double kx = kk * (ax * bx + ay * by + az * bz);
+    const double ky = kk * (2.0 * (ax * ax + ay * ay + az * az) + (dx * bx + dy * by + dz * bz)) / 3.0;
+    const double kz = kk * (dx * ax + dy * ay + dz * az);
+    const double p = ky - kx * kx;
+    const double q = kx * (2.0 * kx * kx - 3.0 * ky) + kz;
+    const double h = q * q + 4.0 * p * p * p;
+    Q.a = v3((float)ax, (float)ay, (float)az);
+    Q.b = v3((float)bx, (float)by, (float)bz);
     Q.c = Q.a * 2.f;
-    Q.d = A - pos;
-    float kk = 1.f / dot(Q.b, Q.b);
-    Q.kx = kk * dot(Q.a, Q.b);
-    Q.ky = kk * (2.f * dot(Q.a, Q.a) + dot(Q.d, Q.b)) / 3.f;
-    Q.kz = kk * dot(Q.d, Q.a);
-    Q.p = Q.ky - Q.kx * Q.kx;
-    float p3 = Q.p * Q.p * Q.p;
-    Q.q = Q.kx * (2.f * Q.kx * Q.kx - 3.f * Q.ky) + Q.kz;
-    Q.h = Q.q * Q.q + 4.f * p3;
-    Q.one = Q.h >= 0.f;
+    Q.d = v3((float)dx, (float)dy, (float)dz);
+    Q.kk = (float)kk;
+    Q.kx = (float)kx;
+    Q.ky = (float)ky;
+    Q.kz = (float)kz;
+    Q.p = (float)p;
+    Q.q = (float)q;
+    Q.one = h >= 0.0;
     Q.use_b = false;
+    Q.clamped = false;
+    Q.z = Q.v = Q.m = Q.n = Q.den = Q.sq = 0.f;
+    float traw;
     if (Q.one) {
-        float hh = sqrtf(Q.h + 1e-8f);
-        Q.x0 = (hh - Q.q) * 0.5f;
-        Q.x1 = -(hh + Q.q) * 0.5f;
-        Q.t = clampf((cbrt_signed(Q.x0) + cbrt_signed(Q.x1)) - Q.kx, 0.f, 1.f);
-        Q.dist2 = quad_d2(Q.d, Q.c, Q.b, Q.t);
+        const double hh = sqrt(h + 1e-8);
+        traw = (cbrt_signed((float)((hh - q) * 0.5)) + cbrt_signed((float)(-(hh + q) * 0.5))) - Q.kx;
+        // p' - p = 2.5e-9 / (p'^2 + p' p + p^2), free of cancellation
+        const float pp = cbrt_signed((float)(p * p * p + 2.5e-9));
+        const float dl = 2.5e-9f / (pp * pp + pp * Q.p + Q.p * Q.p);
+        Q.c1 = 3.f * dl;
+        Q.c0 = 0.f;
+        Q.ep = pp != 0.f ? -3.f * dl * (pp + Q.p) / (pp * pp) : 0.f;  // d(3 dl)/dp
+        Q.eq = 0.f;
     } else {
-        Q.z = sqrtf(-Q.p + 1e-8f);
-        // the 1e-8 regularisers can push |q / (2 p z)| past 1 when h ~ 0, where
-        // the reference's acosf returns NaN; the argument is clamped to [-1, 1]
-        Q.v = acosf(clampf(Q.q / (Q.p * Q.z * 2.f + 1e-8f), -1.f, 1.f)) / 3.f;
+        const double z = sqrt(-p + 1e-8);
+        const double den = p * z * 2.0 + 1e-8;
+        const double arg = q / den;
+        // the regularisers can push |arg| past 1 when h ~ 0, where the reference's
+        // acosf returns NaN; acos is taken of the argument clamped to [-1, 1]
+        Q.clamped = !(arg > -1.0 && arg < 1.0);
+        // acos via asin of the half complement, which double precision keeps exact
+        const float om = (float)fmax(1.0 - arg, 0.0), op = (float)fmax(1.0 + arg, 0.0);
+        const float ac = arg >= 0.0 ? 2.f * asinf(fminf(sqrtf(om * 0.5f), 1.f))
+                                    : 3.14159265358979f - 2.f * asinf(fminf(sqrtf(op * 0.5f), 1.f));
+        Q.z = (float)z;
+        Q.den = (float)den;
+        Q.sq = sqrtf(om * op);
+        Q.v = ac / 3.f;
         Q.m = cosf(Q.v);
         Q.n = sinf(Q.v) * 1.732050808f;
-        float ta = clampf((Q.m + Q.n) * Q.z - Q.kx, 0.f, 1.f);
-        float tb = clampf((-Q.n - Q.m) * Q.z - Q.kx, 0.f, 1.f);
-        float da = quad_d2(Q.d, Q.c, Q.b, ta);
-        float db = quad_d2(Q.d, Q.c, Q.b, tb);
+        const float ra = (Q.m + Q.n) * Q.z - Q.kx, rb = (-Q.n - Q.m) * Q.z - Q.kx;
+        const float da = quad_d2(Q.d, Q.c, Q.b, clampf(ra, 0.f, 1.f));
+        const float db = quad_d2(Q.d, Q.c, Q.b, clampf(rb, 0.f, 1.f));
         Q.use_b = db < da;
-        Q.t = Q.use_b ? tb : ta;
-        Q.dist2 = Q.use_b ? db : da;
+        traw = Q.use_b ? rb : ra;
+        const double kap = 1e-8 + 5e-9 / z, mu = p + 5e-9 / z, dkap = 5e-9 / (2.0 * z * z * z);
+        Q.c1 = -3e-8f;
+        Q.c0 = (float)(-q * kap / mu);
+        Q.ep = (float)(-q * (dkap * mu - kap * (1.0 + dkap)) / (mu * mu));
+        Q.eq = (float)(-kap / mu);
     }
+    Q.root = (Q.one || (Q.use_b && !Q.clamped)) && traw > 0.f && traw < 1.f;
+    float t = clampf(traw, 0.f, 1.f);
+    if (Q.root) {
+        const V3 v = Q.d + (Q.c + Q.b * t) * t;
+        const V3 tg = Q.a + Q.b * t;
+        const float F = dot(v, tg) * Q.kk + Q.c1 * (t + Q.kx) + Q.c0;
+        const float Fp = (2.f * dot(tg, tg) + dot(v, Q.b)) * Q.kk + Q.c1;
+        if (Fp > 0.f) t -= F / Fp;
+        if (!(t > 0.f && t < 1.f)) {  // the root lies past an end: the pick clamps there
+            Q.root = false;
+            traw = t;
+            t = clampf(t, 0.f, 1.f);
+        } else {
+            traw = t;
+        }
+    }
+    Q.traw = traw;
+    Q.t = t;
+    Q.dist2 = quad_d2(Q.d, Q.c, Q.b, t);
     return Q;
 }
 
@@ -337,64 +407,60 @@ __device__ __forceinline__ V3 face_normal(V3 q, V3 p) {
     return v3(gx, gy, gz);
 }
 
-// quadratic Bezier: reverse-mode derivative of the closed-form solve itself
-// (cube roots / acos branch, including its 1e-8 regularisers), i.e. the exact
-// derivative of eval_quad_bezier.
+// quadratic Bezier: exact derivative of eval_quad_bezier.  For refined roots
+// dt/dtheta = -(dF/dtheta) / F'(t) (implicit function theorem on the cubic the
+// closed form solves); for the three-root branch's first candidate the
+// reverse-mode derivative of the closed form (acos branch, its regularisers).
 __device__ __forceinline__ V3 grad_ex_quad_solved(V3 pos, const float* prm, const QuadSolve& Q, float g, float* gp) {
     const float t = Q.t;
     const float r1 = prm[9], r2 = prm[10];
     const V3 v = Q.d + (Q.c + Q.b * t) * t;  // P(t) - pos
-    const V3 Pp = Q.c + 2.f * Q.b * t;       // P'(t)
+    const V3 tg = Q.a + Q.b * t;             // P'(t) / 2
     const float dist = sqrtf(dot(v, v) + 1e-8f);
     const V3 vn = v * (1.f / dist);
-    const float G = dot(v, Pp) / dist - (r2 - r1);  // dSDF/dt at fixed geometry
-    float traw, gq = 0.f, gpp = 0.f, gkx = 0.f;
-    if (Q.one) {
-        traw = (cbrt_signed(Q.x0) + cbrt_signed(Q.x1)) - Q.kx;
-        float gt = (traw > 0.f && traw < 1.f) ? G : 0.f;
-        // d cbrt(x)/dx = |x|^(-2/3) / 3
-        float gx0 = Q.x0 != 0.f ? gt * (1.f / 3.f) * powf(fabsf(Q.x0), -2.f / 3.f) : 0.f;
-        float gx1 = Q.x1 != 0.f ? gt * (1.f / 3.f) * powf(fabsf(Q.x1), -2.f / 3.f) : 0.f;
-        float hh = sqrtf(Q.h + 1e-8f);
-        float ghh = 0.5f * gx0 - 0.5f * gx1;
-        gq = -0.5f * gx0 - 0.5f * gx1;
-        float gh = ghh / (2.f * hh);
-        gq += 2.f * Q.q * gh;
-        gpp = 12.f * Q.p * Q.p * gh;
-        gkx = -gt;
-    } else {
+    const float G = 2.f * dot(v, tg) / dist - (r2 - r1);  // dSDF/dt at fixed geometry
+    const float kk = Q.kk;
+    // direct dependence of v = d + 2 a t + b t^2 at fixed t
+    V3 ga = vn * (2.f * t), gb = vn * (t * t), gd = vn;
+    float gkx = 0.f, gky = 0.f, gkz = 0.f;
+    if (Q.root) {
+        const float gg = dot(v, tg);
+        const float Fp = (2.f * dot(tg, tg) + dot(v, Q.b)) * kk + Q.c1;
+        const float lam = Fp > 0.f ? -G / Fp : 0.f;
+        // g / |b|^2 part of F (kk = 1 / |b|^2)
+        ga = ga + (2.f * t * tg + v) * (lam * kk);
+        gb = gb + (t * t * tg + t * v) * (lam * kk) - Q.b * (2.f * lam * gg * kk * kk);
+        gd = gd + tg * (lam * kk);
+        // regulariser part C(kx, p, q) = c1 (t + kx) + c0
+        const float Cp = lam * (Q.one ? Q.ep * (t + Q.kx) : Q.ep), Cq = lam * Q.eq;
+        gkx = lam * Q.c1 + Cq * (6.f * Q.kx * Q.kx - 3.f * Q.ky) - 2.f * Q.kx * Cp;
+        gky = -3.f * Q.kx * Cq + Cp;
+        gkz = Cq;
+    } else if (!Q.one) {
         const float sg = Q.use_b ? -1.f : 1.f;
-        traw = sg * (Q.m + Q.n) * Q.z - Q.kx;
-        float gt = (traw > 0.f && traw < 1.f) ? G : 0.f;
+        const float gt = (Q.traw > 0.f && Q.traw < 1.f) ? G : 0.f;
         float gm = gt * sg * Q.z, gn = gt * sg * Q.z, gz = gt * sg * (Q.m + Q.n);
         float gv = -sinf(Q.v) * gm + 1.732050808f * cosf(Q.v) * gn;
-        float den = Q.p * Q.z * 2.f + 1e-8f;
-        float arg = Q.q / den;
-        float garg = fabsf(arg) < 1.f ? -gv / (3.f * sqrtf(1.f - arg * arg)) : 0.f;
-        gq = garg / den;
-        float gden = -garg * Q.q / (den * den);
-        gpp = 2.f * Q.z * gden;
+        // d acos(arg) = -d arg / sqrt(1 - arg^2); zero where the argument is clamped
+        float garg = (!Q.clamped && Q.sq > 0.f) ? -gv / (3.f * Q.sq) : 0.f;
+        float gq = garg / Q.den;
+        float gden = -garg * Q.q / (Q.den * Q.den);
+        float gpp = 2.f * Q.z * gden;
         gz += 2.f * Q.p * gden;
         gpp += -gz / (2.f * Q.z);
-        gkx = -gt;
+        // q = 2kx^3 - 3 kx ky + kz, p = ky - kx^2
+        gkx = -gt + gq * (6.f * Q.kx * Q.kx - 3.f * Q.ky) - 2.f * Q.kx * gpp;
+        gky = -3.f * Q.kx * gq + gpp;
+        gkz = gq;
     }
-    // q = 2kx^3 - 3 kx ky + kz, p = ky - kx^2
-    gkx += gq * (6.f * Q.kx * Q.kx - 3.f * Q.ky) - 2.f * Q.kx * gpp;
-    const float gky = -3.f * Q.kx * gq + gpp;
-    const float gkz = gq;
     // kx = kk ab, ky = kk (2 aa + db)/3, kz = kk da, kk = 1/bb
-    const float bb = dot(Q.b, Q.b), kk = 1.f / bb;
     const float ab = dot(Q.a, Q.b), aa = dot(Q.a, Q.a), db = dot(Q.d, Q.b), da = dot(Q.d, Q.a);
     const float gkk = gkx * ab + gky * (2.f * aa + db) / 3.f + gkz * da;
     const float gab = gkx * kk, gaa = gky * kk * (2.f / 3.f), gdb = gky * kk * (1.f / 3.f), gda = gkz * kk;
     const float gbb = -gkk * kk * kk;
-    V3 ga = gab * Q.b + (2.f * gaa) * Q.a + gda * Q.d;
-    V3 gb = gab * Q.a + gdb * Q.d + (2.f * gbb) * Q.b;
-    V3 gd = gdb * Q.b + gda * Q.a;
-    // direct dependence of v = d + c t + b t^2 (c = 2a) at fixed t
-    gd = gd + vn;
-    gb = gb + vn * (t * t);
-    ga = ga + vn * (2.f * t);
+    ga = ga + gab * Q.b + (2.f * gaa) * Q.a + gda * Q.d;
+    gb = gb + gab * Q.a + gdb * Q.d + (2.f * gbb) * Q.b;
+    gd = gd + gdb * Q.b + gda * Q.a;
     // a = B - A, b = A - 2B + C, d = A - pos
     add3(gp + 0, (-ga + gb + gd) * g);
     add3(gp + 3, (ga - 2.f * gb) * g);

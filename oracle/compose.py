@@ -30,9 +30,9 @@ def compose_forward(alphas, colors, density_params, skip_zero=True):
     the one-sided limit).
     """
     N, S = alphas.shape
-    T = torch.ones(N, dtype=alphas.dtype)
-    dens = torch.zeros(N, dtype=alphas.dtype)
-    col = torch.zeros(N, colors.shape[-1], dtype=alphas.dtype)
+    T = torch.ones(N, dtype=alphas.dtype, device=alphas.device)
+    dens = torch.zeros(N, dtype=alphas.dtype, device=alphas.device)
+    col = torch.zeros(N, colors.shape[-1], dtype=alphas.dtype, device=alphas.device)
     for k in range(S - 1, -1, -1):
         a = alphas[:, k]
         nz = (a != 0.0) if skip_zero else torch.ones_like(a, dtype=torch.bool)
@@ -65,17 +65,17 @@ def compose_backward_reference(grad_density, grad_color, alphas, colors, density
     Dc = colors.shape[-1]
     dens, _, col, Tn = compose_forward(alphas, colors, density_params)
     if grad_density is None or grad_density.numel() == 0:
-        grad_density = torch.zeros(N, dtype=alphas.dtype)
+        grad_density = torch.zeros(N, dtype=alphas.dtype, device=alphas.device)
     if grad_color is None or grad_color.numel() == 0:
-        grad_color = torch.zeros(N, Dc, dtype=alphas.dtype)
+        grad_color = torch.zeros(N, Dc, dtype=alphas.dtype, device=alphas.device)
     O = _ONE_PLUS_EPS - Tn
     scale = 1.0 / O
     scaled = col * scale[:, None]
     in_range = (scaled >= 0.0) & (scaled <= 1.0)
     gC = torch.where(in_range, grad_color, torch.zeros_like(grad_color))
-    T = torch.ones(N, dtype=alphas.dtype)
-    d2 = torch.zeros(N, dtype=alphas.dtype)
-    c2 = torch.zeros(N, Dc, dtype=alphas.dtype)
+    T = torch.ones(N, dtype=alphas.dtype, device=alphas.device)
+    d2 = torch.zeros(N, dtype=alphas.dtype, device=alphas.device)
+    c2 = torch.zeros(N, Dc, dtype=alphas.dtype, device=alphas.device)
     g_alpha = torch.zeros_like(alphas)
     g_colors = torch.zeros_like(colors)
     g_dp = torch.zeros_like(density_params)

@@ -56,6 +56,38 @@ def test_render_exact(name, n_rays, n_samples, n_strokes, dev):
     assert out['acc'].max() > 0.5  # rays do hit strokes
 
 
+def _oracle_on(sc, dev):
+    """Oracle arguments as float64 tensors on ``dev`` (the oracle's torch ops run there)."""
+    return tuple(a.to(dev, D64) if torch.is_tensor(a) and a.dtype.is_floating_point else a for a in _oracle_args(sc))
+
+
+def _rows_err(a, b, rows):
+    return rel_err(a[rows], b[rows])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('store', ['auto', False])
+def test_render_palette500_all_strokes(store, dev):
+    """The 500-stroke workload at its full stroke count (200 cubic, 100 quadratic Bezier,
+    200 solids; 512 rays x 128 samples) against the float64 oracle, every pair counted
+    (no conditioning mask): outputs within 1e-5, every parameter gradient within 1e-4,
+    and each stroke type's rows within 1e-4 on their own."""
+    sc = _scene('palette500_65536x128', 512, 128)
+    (rgb, acc, depth), (gs, gc, gd), (g_rgb, g_acc) = _run(sc, dev, store=store)
+    out, rs, rc, rd = ofield.render_field_grads_chunked(g_rgb.to(dev, D64), g_acc.to(dev, D64), *_oracle_on(sc, dev),
+                                                        chunk=32)
+    errs = dict(rgb=rel_err(rgb, out['rgb']), acc=rel_err(acc, out['acc']), depth=rel_err(depth, out['depth']),
+                shape=rel_err(gs, rs), color=rel_err(gc, rc), density=rel_err(gd, rd))
+    types = torch.tensor(sc['types'])
+    for b in sorted(set(sc['types'])):
+        rows = (types == b).nonzero().flatten()
+        errs[f'shape_type{b}'] = _rows_err(gs.cpu(), rs.cpu(), rows)
+    print('palette500 all strokes, rel errors vs float64 oracle:', errs)
+    assert errs['rgb'] < 1e-5 and errs['acc'] < 1e-5 and errs['depth'] < 1e-5, errs
+    assert max(v for k, v in errs.items() if k not in ('rgb', 'acc', 'depth')) < 1e-4, errs
+    assert out['acc'].max() > 0.5
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize('store', [False, True, (6, 40)])
 @pytest.mark.parametrize('color_id', [0, 1])

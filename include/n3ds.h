@@ -222,6 +222,30 @@ int n3ds_render_backward(const float* origins, const float* directions, const fl
                          int64_t workspace_floats, float* grad_shape, float* grad_color, float* grad_density,
                          void* stream);
 
+/* Allocation-free variant of n3ds_render_backward.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as n3ds_render_backward without the workspace:
+ * the per-block partial gradient rows live in a fixed device array of the
+ * library (n3ds_render_backward_noalloc_scratch_floats() floats = 32 MiB, one
+ * copy per GPU, held until the process ends).  It allocates nothing and
+ * rejects n_strokes * (shape_width + color_width + 1) beyond that size
+ * (N3DS_EINVAL); the persistent block count is lowered to fit the scratch,
+ * which changes only the float summation order of the gradients.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller. */
+int64_t n3ds_render_backward_noalloc_scratch_floats(void);
+int n3ds_render_backward_noalloc(const float* origins, const float* directions, const float* viewdirs,
+                                 const float* radii, const float* tdist, const float* table,
+                                 const float* color_params, const float* density_params, int64_t n_rays,
+                                 int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
+                                 float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
+                                 float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
+                                 const float* grad_rgb, const float* grad_acc, const float* grad_depth,
+                                 const float* state, const int* hit_dir, const int* hit_buf, const float* pair_buf,
+                                 int cap_hits, int cap_pairs, float* grad_shape, float* grad_color,
+                                 float* grad_density, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batches of independent scenes.  No counterpart in the reference, which
  * trains one scene per process (train.py:127-144): many scenes with uneven

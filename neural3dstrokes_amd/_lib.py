@@ -48,6 +48,10 @@ SIGNATURES = {
     'n3ds_render_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
                                     _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P,
                                     _P, _P]),
+    'n3ds_render_backward_noalloc_scratch_floats': (_I64, []),
+    'n3ds_render_backward_noalloc': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F,
+                                            _I32, _I32, _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
+                                            _P, _P, _P, _P]),
     'n3ds_render_batch_plan': (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32]),
     'n3ds_render_batch_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32, _I32,
                                          _I32, _F, _I32, _I32, _I32, _F, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32,

@@ -33,6 +33,8 @@
 // Sample depths must be non-decreasing along a ray (as the reference's
 // stratified samplers produce); a ray whose depths are not sorted falls back
 // to evaluating every hit stroke on all its samples (same result, slower).
+#include <mutex>
+
 #include "n3ds_capi.h"
 #include "n3ds_stroke_common.h"
 
@@ -2299,17 +2301,21 @@ N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_
     return max_persistent_blocks(n_rays) * n_strokes * PG;
 }
 
-N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directions, const float* viewdirs,
-                                     const float* radii, const float* tdist, const float* table,
-                                     const float* color_params, const float* density_params, int64_t n_rays,
-                                     int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
-                                     float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
-                                     float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
-                                     const float* grad_rgb, const float* grad_acc, const float* grad_depth,
-                                     const float* state, const int* hit_dir, const int* hit_buf,
-                                     const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
-                                     int64_t workspace_floats, float* grad_shape, float* grad_color,
-                                     float* grad_density, void* stream) {
+// Backward body shared by n3ds_render_backward (caller workspace) and
+// n3ds_render_backward_noalloc (the library's fixed scratch).  With fit_blocks
+// the persistent block count is lowered until the partial rows fit the
+// workspace; the results depend on the block count only through the float
+// summation order of the gradients.
+static int render_backward_impl(const float* origins, const float* directions, const float* viewdirs,
+                                const float* radii, const float* tdist, const float* table, const float* color_params,
+                                const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
+                                int shape_width, int color_width, int color_id, float sdf_delta,
+                                int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
+                                const float* bg_color, float cull_eps, int adjoint_exact, const float* grad_rgb,
+                                const float* grad_acc, const float* grad_depth, const float* state, const int* hit_dir,
+                                const int* hit_buf, const float* pair_buf, int cap_hits, int cap_pairs,
+                                float* workspace, int64_t workspace_floats, bool fit_blocks, float* grad_shape,
+                                float* grad_color, float* grad_density, void* stream) {
     int st = render_check(n_rays, n_samples, n_strokes, shape_width, color_width, color_id);
     if (st) return st;
     st = device_info();
@@ -2343,6 +2349,12 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
         if (fit < nb) nb = fit;
     }
     const int64_t tot = (int64_t)n_strokes * PG;
+    if (fit_blocks) {
+        N3DS_REQUIRE(tot <= workspace_floats,
+                     "%d strokes x %d gradient columns exceed the %lld-float scratch of the allocation-free backward",
+                     n_strokes, PG, (long long)workspace_floats);
+        if (bs.use_lds && nb * tot > workspace_floats) nb = workspace_floats / tot;
+    }
     const int64_t ws_need = bs.use_lds ? nb * tot : tot;
     N3DS_REQUIRE(workspace_floats >= ws_need, "workspace has %lld floats, need %lld", (long long)workspace_floats,
                  (long long)ws_need);
@@ -2364,6 +2376,75 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
                        grad_density);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;
+}
+
+N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directions, const float* viewdirs,
+                                     const float* radii, const float* tdist, const float* table,
+                                     const float* color_params, const float* density_params, int64_t n_rays,
+                                     int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
+                                     float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
+                                     float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
+                                     const float* grad_rgb, const float* grad_acc, const float* grad_depth,
+                                     const float* state, const int* hit_dir, const int* hit_buf,
+                                     const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
+                                     int64_t workspace_floats, float* grad_shape, float* grad_color,
+                                     float* grad_density, void* stream) {
+    return render_backward_impl(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
+                                n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
+                                use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                                adjoint_exact, grad_rgb, grad_acc, grad_depth, state, hit_dir, hit_buf, pair_buf,
+                                cap_hits, cap_pairs, workspace, workspace_floats, false, grad_shape, grad_color,
+                                grad_density, stream);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free backward: the partial gradient rows live in a fixed
+// file-scope device array, one copy per GPU, held until the process ends
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_SCRATCH_FLOATS = int64_t(8) << 20;  // 32 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
+// device address of `scratch` per device, from hipGetSymbolAddress with that device current
+static float* g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(float** out) {
+    int dev = 0;
+    N3DS_HIP_CHECK(hipGetDevice(&dev));
+    N3DS_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device %d beyond the allocation-free scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev] == nullptr) {
+        void* p = nullptr;
+        N3DS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        N3DS_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int64_t n3ds_render_backward_noalloc_scratch_floats(void) { return NOALLOC_SCRATCH_FLOATS; }
+
+N3DS_EXPORT int n3ds_render_backward_noalloc(const float* origins, const float* directions, const float* viewdirs,
+                                             const float* radii, const float* tdist, const float* table,
+                                             const float* color_params, const float* density_params, int64_t n_rays,
+                                             int n_samples, int n_strokes, int shape_width, int color_width,
+                                             int color_id, float sdf_delta, int use_laplace_transform,
+                                             int inv_scale_radius, int cone, float coord_div, const float* bg_color,
+                                             float cull_eps, int adjoint_exact, const float* grad_rgb,
+                                             const float* grad_acc, const float* grad_depth, const float* state,
+                                             const int* hit_dir, const int* hit_buf, const float* pair_buf,
+                                             int cap_hits, int cap_pairs, float* grad_shape, float* grad_color,
+                                             float* grad_density, void* stream) {
+    float* ws = nullptr;
+    const int st = scratch_address(&ws);
+    if (st) return st;
+    return render_backward_impl(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
+                                n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
+                                use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                                adjoint_exact, grad_rgb, grad_acc, grad_depth, state, hit_dir, hit_buf, pair_buf,
+                                cap_hits, cap_pairs, ws, NOALLOC_SCRATCH_FLOATS, true, grad_shape, grad_color,
+                                grad_density, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -578,26 +578,47 @@ def hbm_traffic(workload, kernel):
     return rec.get('bytes_per_launch', {}).get(kernel)
 
 
-def cpu_baseline(sc, n_rays, g_rgb, g_acc):
-    """Float32 CPU oracle (plain per-op math, no culling) fwd+bwd on the first n_rays rays."""
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on (its affinity mask,
+    which follows the cgroup's cpuset), capped by OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1)
+    omp = os.environ.get('OMP_NUM_THREADS')
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(sc, n_rays, g_rgb, g_acc, reps=3, chunk=128):
+    """Float32 CPU oracle (plain per-op math, no culling) fwd+bwd on the first n_rays rays,
+    in chunks of ``chunk`` rays (gradients add up over chunks); 1 warmup on one chunk, then
+    ``reps`` timed passes over all n_rays."""
     import torch
     from oracle import field as ofield
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     torch.set_num_threads(threads)
     d = torch.float32
     args = [sc[k][:n_rays].to(d) for k in ('origins', 'directions', 'viewdirs', 'radii', 'tdist')]
-    call = lambda: ofield.render_field_grads(g_rgb[:n_rays].cpu(), g_acc[:n_rays].cpu(), *args, sc['stroke_ids'],
-                                             sc['shape_params'].to(d), sc['color_params'].to(d),
-                                             sc['density_params'].to(d))
-    call()
-    t0 = time.perf_counter()
-    call()
-    dt = time.perf_counter() - t0
+
+    def call(r1):
+        for r0 in range(0, r1, chunk):
+            sl = slice(r0, min(r1, r0 + chunk))
+            ofield.render_field_grads(g_rgb[sl].cpu(), g_acc[sl].cpu(), *[a[sl] for a in args], sc['stroke_ids'],
+                                      sc['shape_params'].to(d), sc['color_params'].to(d), sc['density_params'].to(d))
+
+    call(min(chunk, n_rays))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        call(n_rays)
+        times.append(time.perf_counter() - t0)
+    dt = sum(times) / len(times)
     n = n_rays * (sc['tdist'].shape[1] - 1)
-    return dict(value=n / dt, unit='ray-samples/s', cores=threads, kind='port',
+    return dict(value=n / dt, unit='ray-samples/s', cores=threads, kind='port', host_cpus=os.cpu_count(),
                 sample=f'{n_rays} rays x {sc["tdist"].shape[1] - 1} samples x {len(sc["stroke_ids"])} strokes '
-                       f'(first rays of the workload), float32 CPU oracle fwd+bwd (autograd), 1 warmup + 1 timed',
-                seconds=dt)
+                       f'(first rays of the workload, {n_rays * (sc["tdist"].shape[1] - 1) * len(sc["stroke_ids"])} '
+                       f'(sample, stroke) pairs), float32 CPU oracle fwd+bwd (autograd) in {chunk}-ray chunks, '
+                       f'1 warmup chunk + {reps} timed passes (mean)',
+                seconds_per_pass=times)
 
 
 def main():
@@ -610,9 +631,10 @@ def main():
     ap.add_argument('--adjoint', default='exact', choices=['exact', 'reference'])
     ap.add_argument('--plain-rays', type=int, default=8192, help='rays for the plain per-op comparator (0: skip)')
     ap.add_argument('--plain-chunk', type=int, default=1024)
-    ap.add_argument('--cpu-rays', type=int, default=256, help='rays for the CPU baseline (0: skip)')
+    ap.add_argument('--cpu-rays', type=int, default=512, help='rays for the CPU baseline (0: skip)')
+    ap.add_argument('--cpu-reps', type=int, default=3, help='timed passes of the CPU baseline')
     ap.add_argument('--fused-only', action='store_true', help='only the fused steps (for profiler runs)')
-    ap.add_argument('--acc-rays', type=int, default=4096, help='rays for the float64 accuracy check (0: skip)')
+    ap.add_argument('--acc-rays', type=int, default=8192, help='rays for the float64 accuracy check (0: skip)')
     ap.add_argument('--fd-params', type=int, default=32, help='parameters for the finite-difference check (0: skip)')
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
     ap.add_argument('--texture', type=int, default=1, help='image-texture block (0: skip)')
@@ -739,10 +761,13 @@ def main():
                  T['shape_params'], T['color_params'], T['density_params'], gr[sl], ga[sl])
         plain.fwd_bwd(*pargs, chunk=args.plain_chunk)  # warmup
         torch.cuda.synchronize()
-        p0 = time.perf_counter()
-        prgb, pacc, pgs, pgc, pgd = plain.fwd_bwd(*pargs, chunk=args.plain_chunk)
-        torch.cuda.synchronize()
-        pdt = time.perf_counter() - p0
+        p_times = []
+        for _ in range(3):
+            p0 = time.perf_counter()
+            prgb, pacc, pgs, pgc, pgd = plain.fwd_bwd(*pargs, chunk=args.plain_chunk)
+            torch.cuda.synchronize()
+            p_times.append(time.perf_counter() - p0)
+        pdt = sum(p_times) / len(p_times)
         plain_value = pr * n / pdt
         # fused on the same rays for the accuracy comparison
         spf = T['shape_params'].clone().requires_grad_(True)
@@ -753,7 +778,7 @@ def main():
                                               adjoint=args.adjoint)
         torch.autograd.backward([frgb, facc], [gr[sl], ga[sl]])
         out['plain'] = {'value': plain_value, 'unit': 'ray-samples/s', 'rays': pr, 'chunk_rays': args.plain_chunk,
-                        's_per_call': pdt,
+                        's_per_call': pdt, 's_per_call_reps': p_times,
                         'speedup_fused_vs_plain': value / plain_value,
                         'rel_err_fused_vs_plain': {'rgb': rel(frgb, prgb), 'acc': rel(facc, pacc),
                                                    'grad_shape': rel(spf.grad, pgs), 'grad_color': rel(cpf.grad, pgc),
@@ -761,7 +786,10 @@ def main():
         if args.acc_rays > 0:
             out['accuracy'] = accuracy_block(args, sc, T, ids, gr, ga, n)
     if args.fd_params > 0:
+        # h = 1e-4 as the workload states; h = 1e-6 crosses fewer kinks of the piecewise alpha
+        # transfer (its clamps at 0 and 0.9999) and of the curve-parameter selection
         out['finite_differences'] = fd_block(sc, T, ids, gr, ga, n_params=args.fd_params)
+        out['finite_differences_h1e-6'] = fd_block(sc, T, ids, gr, ga, n_params=args.fd_params, h=1e-6)
     if args.grid:
         out['hash_grid'] = grid_block()
     if args.texture:
@@ -773,7 +801,7 @@ def main():
     if args.multiscene:
         out['multiscene'] = multiscene_block()
     if args.cpu_rays > 0:
-        out['cpu_baseline'] = cpu_baseline(sc, min(args.cpu_rays, R), g_rgb, g_acc)
+        out['cpu_baseline'] = cpu_baseline(sc, min(args.cpu_rays, R), g_rgb, g_acc, reps=args.cpu_reps)
     print(json.dumps(out))
 
 

@@ -161,7 +161,10 @@ int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* sh
  * composite state the backward reads.  counters [25] uint64 (or NULL) receive
  * per-base-type counts of evaluated and alpha>0 (sample, stroke) pairs.
  * cone != 0: sample radius = radii * t; coord_div divides sample coordinates
- * (models.py:663 warp); bg_color is a host pointer to 3 floats; cull_eps is the
+ * (models.py:663 warp); bg_color is a host pointer to 3 floats, the background
+ * of every ray unless bg_rgbs (device [R,3], per-ray backgrounds as drawn by
+ * models.py:213-222, or NULL) is given; opaque_background makes the last
+ * interval infinitely wide (render.py:35-40: its alpha is 1); cull_eps is the
  * alpha below which the Laplace transfer's tail is culled (linear transfer:
  * culling is exact).
  * Optional activations kept for the backward (hit_dir == NULL: none, the
@@ -180,7 +183,8 @@ int n3ds_render_forward(const float* origins, const float* directions, const flo
                         const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
                         int shape_width, int color_width, int color_id, float sdf_delta,
                         int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
-                        const float* bg_color, float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
+                        const float* bg_color, const float* bg_rgbs, int opaque_background,
+                        float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
                         float* state_out, unsigned long long* counters, int* hit_dir, int* hit_buf,
                         float* pair_buf, int cap_hits, int cap_pairs, void* workspace, int64_t workspace_bytes,
                         void* stream);
@@ -216,7 +220,8 @@ int n3ds_render_backward(const float* origins, const float* directions, const fl
                          const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
                          int shape_width, int color_width, int color_id, float sdf_delta,
                          int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
-                         const float* bg_color, float cull_eps, int adjoint_exact, const float* grad_rgb,
+                         const float* bg_color, const float* bg_rgbs, int opaque_background,
+                        float cull_eps, int adjoint_exact, const float* grad_rgb,
                          const float* grad_acc, const float* grad_depth, const float* state, const int* hit_dir,
                          const int* hit_buf, const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
                          int64_t workspace_floats, float* grad_shape, float* grad_color, float* grad_density,
@@ -240,7 +245,8 @@ int n3ds_render_backward_noalloc(const float* origins, const float* directions, 
                                  const float* color_params, const float* density_params, int64_t n_rays,
                                  int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
                                  float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
-                                 float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
+                                 float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background,
+                        float cull_eps, int adjoint_exact,
                                  const float* grad_rgb, const float* grad_acc, const float* grad_depth,
                                  const float* state, const int* hit_dir, const int* hit_buf, const float* pair_buf,
                                  int cap_hits, int cap_pairs, float* grad_shape, float* grad_color,
@@ -278,7 +284,8 @@ int n3ds_render_batch_forward(const float* origins, const float* directions, con
                               const float* density_params, const int32_t* scenes_host, const int32_t* scenes,
                               int n_scenes, int64_t n_rays, int n_samples, int n_strokes, int shape_width,
                               int color_width, int color_id, float sdf_delta, int use_laplace_transform,
-                              int inv_scale_radius, int cone, float coord_div, const float* bg_color, float cull_eps,
+                              int inv_scale_radius, int cone, float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background,
+                        float cull_eps,
                               float* rgb_out, float* acc_out, float* depth_out, float* state_out,
                               unsigned long long* counters, int* hit_dir, int* hit_buf, float* pair_buf, int cap_hits,
                               int cap_pairs, void* workspace, int64_t workspace_bytes, void* stream);
@@ -298,7 +305,8 @@ int n3ds_render_batch_backward(const float* origins, const float* directions, co
                                const float* density_params, const int32_t* scenes_host, const int32_t* scenes,
                                int n_scenes, int64_t n_rays, int n_samples, int n_strokes, int shape_width,
                                int color_width, int color_id, float sdf_delta, int use_laplace_transform,
-                               int inv_scale_radius, int cone, float coord_div, const float* bg_color, float cull_eps,
+                               int inv_scale_radius, int cone, float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background,
+                        float cull_eps,
                                int adjoint_exact, const float* grad_rgb, const float* grad_acc,
                                const float* grad_depth, const float* state, const int* hit_dir, const int* hit_buf,
                                const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,

@@ -41,25 +41,25 @@ SIGNATURES = {
     'n3ds_render_table_floats': (_I64, [_I32]),
     'n3ds_render_prepare': (_I32, [_P, _P, _P, _I32, _I32, _P, _P]),
     'n3ds_render_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
-                                   _I32, _F, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P]),
+                                   _I32, _F, _P, _P, _I32, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P]),
     'n3ds_render_forward_workspace_bytes': (_I64, [_I64, _I32]),
     'n3ds_render_tuning': (_I32, [_I32, _I32]),
     'n3ds_render_backward_workspace_floats': (_I64, [_I64, _I32, _I32, _I32]),
-    'n3ds_render_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
-                                    _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P,
-                                    _P, _P]),
+    'n3ds_render_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32,
+                                    _I32, _I32, _F, _P, _P, _I32, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
+                                    _P, _I64, _P, _P, _P, _P]),
     'n3ds_render_backward_noalloc_scratch_floats': (_I64, []),
     'n3ds_render_backward_noalloc': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F,
-                                            _I32, _I32, _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
-                                            _P, _P, _P, _P]),
+                                            _I32, _I32, _I32, _F, _P, _P, _I32, _F, _I32, _P, _P, _P, _P, _P, _P, _P,
+                                            _I32, _I32, _P, _P, _P, _P]),
     'n3ds_render_batch_plan': (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32]),
     'n3ds_render_batch_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32, _I32,
-                                         _I32, _F, _I32, _I32, _I32, _F, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32,
-                                         _I32, _P, _I64, _P]),
+                                         _I32, _F, _I32, _I32, _I32, _F, _P, _P, _I32, _F, _P, _P, _P, _P, _P, _P, _P,
+                                         _P, _I32, _I32, _P, _I64, _P]),
     'n3ds_render_batch_backward_workspace_floats': (_I64, [_P, _I32, _I32, _I32, _I32, _I32]),
     'n3ds_render_batch_backward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32, _I32,
-                                          _I32, _F, _I32, _I32, _I32, _F, _P, _F, _I32, _P, _P, _P, _P, _P, _P, _P,
-                                          _I32, _I32, _P, _I64, _P, _P, _P, _P]),
+                                          _I32, _F, _I32, _I32, _I32, _F, _P, _P, _I32, _F, _I32, _P, _P, _P, _P, _P,
+                                          _P, _P, _I32, _I32, _P, _I64, _P, _P, _P, _P]),
     'n3ds_sample_intervals': (_I32, [_P, _P, _P, _P, _P, _I32, _F, _I64, _I32, _I32, _F, _F, _P]),
     'n3ds_max_dilate_weights': (_I32, [_P, _P, _P, _P, _I64, _I32, _F, _F, _F, _I32, _P]),
     'n3ds_texture_pack_floats': (_I64, [_I64, _I64, _I64]),

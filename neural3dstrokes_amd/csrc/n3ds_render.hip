@@ -84,6 +84,8 @@ struct RenderArgs {
     int laplace, isr, cone, exact;
     float coord_div;
     float bg0, bg1, bg2;
+    const float* bgs;  // per-ray background colours [R, 3] (nullptr: bg0..bg2 for every ray)
+    int opaque;        // opaque_background: the last interval is infinitely wide
     float tail;  // culling margin multiplier
     // hits and per-pair values the forward keeps for the adjoint (hdir == nullptr: none).
     // hdir [R, nch, 3]: per 64-stroke chunk (hit offset or -1 if not kept, pair offset, hits);
@@ -1190,10 +1192,12 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
     bool valid[NS];
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
     if (!sorted) unsorted_range<NS>(A, rc, ts, valid);
-    float T[NS], Dn[NS];
+    // T: transmittance past the strokes composited so far; U = 1 - T kept as the sum of the
+    // weights (no cancellation when T ~ 1, where the normalisation 1 / (1e-6 + 1 - T) is steep)
+    float T[NS], U[NS], Dn[NS];
     V3 C[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) { T[s] = 1.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
+    for (int s = 0; s < NS; s++) { T[s] = 1.f; U[s] = 0.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
     unsigned long long my_pos = 0, my_ev = 0, my_vis = 0;  // lane b counts stroke type b (counting pass)
     // cursors into this ray's region of the kept hits / pair values; once a group of
     // chunks does not fit, it and every later chunk are left to the adjoint to recompute
@@ -1376,6 +1380,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
                             const V3 col = (CID == 1) ? v3(w.pb[PBUF + q], w.pb[2 * PBUF + q], w.pb[3 * PBUF + q]) : cu;
                             const float wt = a * T[s];
                             T[s] *= (1.f - a);
+                            U[s] += wt;
                             Dn[s] += dk * wt;
                             C[s] = C[s] + col * wt;
                         }
@@ -1406,7 +1411,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const int i = lane + 64 * s;
-        const float scl = 1.f / (1.f + 1e-6f - T[s]);
+        const float scl = 1.f / (1e-6f + U[s]);
         rgbn[s][0] = clampf(C[s].x * scl, 0.f, 1.f);
         rgbn[s][1] = clampf(C[s].y * scl, 0.f, 1.f);
         rgbn[s][2] = clampf(C[s].z * scl, 0.f, 1.f);
@@ -1416,7 +1421,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
             state[N + si] = C[s].x;
             state[2 * N + si] = C[s].y;
             state[3 * N + si] = C[s].z;
-            state[4 * N + si] = T[s];
+            state[4 * N + si] = U[s];
             sd[s] = Dn[s] * ((td[i + 1] - td[i]) * dn);
         } else {
             sd[s] = 0.f;
@@ -1435,7 +1440,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
         }
         const float excl = carry + v - sd[s];
         const float trans = expf(-excl);
-        const float al = 1.f - expf(-sd[s]);
+        const float al = (A.opaque && lane + 64 * s == A.n - 1) ? 1.f : 1.f - expf(-sd[s]);
         const float wt = valid[s] ? al * trans : 0.f;
         carry += __shfl(v, 63, 64);
         acc += wt;
@@ -1451,9 +1456,10 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
     dsum = wave_sum(dsum);
     if (lane == 0) {
         const float bgw = fmaxf(1.f - acc, 0.f);
-        rgb_out[3 * ray + 0] = r0 + bgw * A.bg0;
-        rgb_out[3 * ray + 1] = r1 + bgw * A.bg1;
-        rgb_out[3 * ray + 2] = r2 + bgw * A.bg2;
+        const V3 bgc = A.bgs ? ld3(A.bgs + 3 * ray) : v3(A.bg0, A.bg1, A.bg2);
+        rgb_out[3 * ray + 0] = r0 + bgw * bgc.x;
+        rgb_out[3 * ray + 1] = r1 + bgw * bgc.y;
+        rgb_out[3 * ray + 2] = r2 + bgw * bgc.z;
         acc_out[ray] = acc;
         float dep = dsum / fmaxf(acc, 1.1920929e-07f);
         if (isnan(dep)) dep = INFINITY;
@@ -1557,7 +1563,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
         const float* td = A.tdist + ray * (A.n + 1);
         const float dn = sqrtf(rc.d.x * rc.d.x + rc.d.y * rc.d.y + rc.d.z * rc.d.z);
         // ---- ray-march adjoint from the saved per-sample state
-        float Dt[NS], Tn[NS], sdl[NS], delta_s[NS];
+        float Dt[NS], Tn[NS], Un[NS], sdl[NS], delta_s[NS];
         V3 Ct[NS], cn[NS];
         bool inr[NS][3];
 #pragma unroll
@@ -1567,12 +1573,13 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 const int64_t si = ray * A.n + i;
                 Dt[s] = state[si];
                 Ct[s] = v3(state[N + si], state[2 * N + si], state[3 * N + si]);
-                Tn[s] = state[4 * N + si];
+                Un[s] = state[4 * N + si];
+                Tn[s] = 1.f - Un[s];
                 delta_s[s] = (td[i + 1] - td[i]) * dn;
             } else {
-                Dt[s] = 0.f; Ct[s] = v3s(0.f); Tn[s] = 1.f; delta_s[s] = 0.f;
+                Dt[s] = 0.f; Ct[s] = v3s(0.f); Tn[s] = 1.f; Un[s] = 0.f; delta_s[s] = 0.f;
             }
-            const float scl = 1.f / (1.f + 1e-6f - Tn[s]);
+            const float scl = 1.f / (1e-6f + Un[s]);
             const V3 sv = Ct[s] * scl;
             inr[s][0] = 0.f <= sv.x && sv.x <= 1.f;
             inr[s][1] = 0.f <= sv.y && sv.y <= 1.f;
@@ -1592,7 +1599,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
             }
             const float excl = carry + v - sdl[s];
             trv[s] = expf(-excl);
-            const float al = 1.f - expf(-sdl[s]);
+            const float al = (A.opaque && lane + 64 * s == A.n - 1) ? 1.f : 1.f - expf(-sdl[s]);
             wv[s] = valid[s] ? al * trv[s] : 0.f;
             carry += __shfl(v, 63, 64);
             acc += wv[s];
@@ -1603,7 +1610,8 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
         const V3 grgb = ld3(g_rgb + 3 * ray);
         const float gacc_r = g_acc ? g_acc[ray] : 0.f;
         const float gdep = g_depth ? g_depth[ray] : 0.f;
-        const float bgdot = (1.f - acc >= 0.f) ? grgb.x * A.bg0 + grgb.y * A.bg1 + grgb.z * A.bg2 : 0.f;
+        const V3 bgc = A.bgs ? ld3(A.bgs + 3 * ray) : v3(A.bg0, A.bg1, A.bg2);
+        const float bgdot = (1.f - acc >= 0.f) ? dot(grgb, bgc) : 0.f;
         const float accc = fmaxf(acc, 1.1920929e-07f);
         const float dep = dsum / accc;
         const bool dep_in = dep >= td[0] && dep <= td[A.n];
@@ -1625,7 +1633,8 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 if (lane + off < 64) v += u;
             }
             const float excl = suf_in + v - wv[s] * gw[s];
-            const float tnext = trv[s] * expf(-sdl[s]);
+            // transmittance past the sample (zero past an opaque last interval)
+            const float tnext = (A.opaque && lane + 64 * s == A.n - 1) ? 0.f : trv[s] * expf(-sdl[s]);
             gsig[s] = delta_s[s] * (tnext * gw[s] - excl);
             suf_in += __shfl(v, 0, 64);
         }
@@ -1637,7 +1646,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
             gD[s] = valid[s] ? gsig[s] : 0.f;
             const V3 gcv = grgb * wv[s];
             gC[s] = v3(inr[s][0] ? gcv.x : 0.f, inr[s][1] ? gcv.y : 0.f, inr[s][2] ? gcv.z : 0.f);
-            O[s] = 1.f + 1e-6f - Tn[s];
+            O[s] = 1e-6f + Un[s];
             iOs[s] = 1.f / O[s];
             Tp[s] = 1.f;
             D2[s] = 0.f;
@@ -2153,12 +2162,14 @@ static int render_check(int64_t R, int n, int S, int Pmax, int Q, int cid) {
 static RenderArgs make_args(const float* origins, const float* dirs, const float* viewdirs, const float* radii,
                             const float* tdist, const float* table, const float* cparams, const float* dparams,
                             int64_t R, int n, int S, int Pmax, int Q, int cid, float delta, int laplace, int isr,
-                            int cone, float coord_div, const float* bg, float cull_eps, int exact) {
+                            int cone, float coord_div, const float* bg, const float* bg_rgbs, int opaque,
+                            float cull_eps, int exact) {
     RenderArgs A;
     A.origins = origins; A.dirs = dirs; A.viewdirs = viewdirs; A.radii = radii; A.tdist = tdist; A.table = table;
     A.cparams = cparams; A.dparams = dparams; A.R = R; A.n = n; A.S = S; A.Q = Q; A.Pmax = Pmax; A.cid = cid;
     A.delta = delta; A.laplace = laplace; A.isr = isr; A.cone = cone; A.exact = exact;
     A.coord_div = coord_div; A.bg0 = bg[0]; A.bg1 = bg[1]; A.bg2 = bg[2];
+    A.bgs = bg_rgbs; A.opaque = opaque;
     if (laplace) {
         float eps = cull_eps > 0.f ? cull_eps : 1e-7f;
         A.tail = fmaxf(1.f, 0.5f * logf(0.5f / eps)) * 1.001f;
@@ -2262,7 +2273,7 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
                                     const float* color_params, const float* density_params, int64_t n_rays,
                                     int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
                                     float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
-                                    float coord_div, const float* bg_color, float cull_eps, float* rgb_out,
+                                    float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background, float cull_eps, float* rgb_out,
                                     float* acc_out, float* depth_out, float* state_out,
                                     unsigned long long* counters, int* hit_dir, int* hit_buf, float* pair_buf,
                                     int cap_hits, int cap_pairs, void* workspace, int64_t workspace_bytes,
@@ -2272,7 +2283,7 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     if (n_rays == 0) return N3DS_OK;
     RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                              n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
-                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps, 0);
+                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, bg_rgbs, opaque_background, cull_eps, 0);
     st = set_keep(A, hit_dir, hit_buf, pair_buf, cap_hits, cap_pairs);
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
@@ -2311,7 +2322,7 @@ static int render_backward_impl(const float* origins, const float* directions, c
                                 const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
                                 int shape_width, int color_width, int color_id, float sdf_delta,
                                 int use_laplace_transform, int inv_scale_radius, int cone, float coord_div,
-                                const float* bg_color, float cull_eps, int adjoint_exact, const float* grad_rgb,
+                                const float* bg_color, const float* bg_rgbs, int opaque_background, float cull_eps, int adjoint_exact, const float* grad_rgb,
                                 const float* grad_acc, const float* grad_depth, const float* state, const int* hit_dir,
                                 const int* hit_buf, const float* pair_buf, int cap_hits, int cap_pairs,
                                 float* workspace, int64_t workspace_floats, bool fit_blocks, float* grad_shape,
@@ -2323,7 +2334,7 @@ static int render_backward_impl(const float* origins, const float* directions, c
     if (n_rays == 0 || n_strokes == 0) return N3DS_OK;
     RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                              n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
-                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, bg_rgbs, opaque_background, cull_eps,
                              adjoint_exact);
     st = set_keep(A, const_cast<int*>(hit_dir), const_cast<int*>(hit_buf), const_cast<float*>(pair_buf), cap_hits,
                   cap_pairs);
@@ -2383,7 +2394,7 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
                                      const float* color_params, const float* density_params, int64_t n_rays,
                                      int n_samples, int n_strokes, int shape_width, int color_width, int color_id,
                                      float sdf_delta, int use_laplace_transform, int inv_scale_radius, int cone,
-                                     float coord_div, const float* bg_color, float cull_eps, int adjoint_exact,
+                                     float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background, float cull_eps, int adjoint_exact,
                                      const float* grad_rgb, const float* grad_acc, const float* grad_depth,
                                      const float* state, const int* hit_dir, const int* hit_buf,
                                      const float* pair_buf, int cap_hits, int cap_pairs, float* workspace,
@@ -2391,7 +2402,7 @@ N3DS_EXPORT int n3ds_render_backward(const float* origins, const float* directio
                                      float* grad_density, void* stream) {
     return render_backward_impl(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                                 n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
-                                use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                                use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, bg_rgbs, opaque_background, cull_eps,
                                 adjoint_exact, grad_rgb, grad_acc, grad_depth, state, hit_dir, hit_buf, pair_buf,
                                 cap_hits, cap_pairs, workspace, workspace_floats, false, grad_shape, grad_color,
                                 grad_density, stream);
@@ -2430,7 +2441,7 @@ N3DS_EXPORT int n3ds_render_backward_noalloc(const float* origins, const float* 
                                              const float* color_params, const float* density_params, int64_t n_rays,
                                              int n_samples, int n_strokes, int shape_width, int color_width,
                                              int color_id, float sdf_delta, int use_laplace_transform,
-                                             int inv_scale_radius, int cone, float coord_div, const float* bg_color,
+                                             int inv_scale_radius, int cone, float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background,
                                              float cull_eps, int adjoint_exact, const float* grad_rgb,
                                              const float* grad_acc, const float* grad_depth, const float* state,
                                              const int* hit_dir, const int* hit_buf, const float* pair_buf,
@@ -2441,7 +2452,7 @@ N3DS_EXPORT int n3ds_render_backward_noalloc(const float* origins, const float* 
     if (st) return st;
     return render_backward_impl(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                                 n_rays, n_samples, n_strokes, shape_width, color_width, color_id, sdf_delta,
-                                use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                                use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, bg_rgbs, opaque_background, cull_eps,
                                 adjoint_exact, grad_rgb, grad_acc, grad_depth, state, hit_dir, hit_buf, pair_buf,
                                 cap_hits, cap_pairs, ws, NOALLOC_SCRATCH_FLOATS, true, grad_shape, grad_color,
                                 grad_density, stream);
@@ -2579,7 +2590,7 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
                                           const int32_t* scenes_host, const int32_t* scenes, int n_scenes,
                                           int64_t n_rays, int n_samples, int n_strokes, int shape_width,
                                           int color_width, int color_id, float sdf_delta, int use_laplace_transform,
-                                          int inv_scale_radius, int cone, float coord_div, const float* bg_color,
+                                          int inv_scale_radius, int cone, float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background,
                                           float cull_eps, float* rgb_out, float* acc_out, float* depth_out,
                                           float* state_out, unsigned long long* counters, int* hit_dir,
                                           int* hit_buf, float* pair_buf, int cap_hits, int cap_pairs,
@@ -2593,7 +2604,7 @@ N3DS_EXPORT int n3ds_render_batch_forward(const float* origins, const float* dir
     if (n_rays == 0) return N3DS_OK;
     RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                              n_rays, n_samples, s_max, shape_width, color_width, color_id, sdf_delta,
-                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps, 0);
+                             use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, bg_rgbs, opaque_background, cull_eps, 0);
     A.scenes = scenes;
     A.n_scenes = n_scenes;
     st = set_keep(A, hit_dir, hit_buf, pair_buf, cap_hits, cap_pairs);
@@ -2652,7 +2663,7 @@ N3DS_EXPORT int n3ds_render_batch_backward(const float* origins, const float* di
                                            const int32_t* scenes_host, const int32_t* scenes, int n_scenes,
                                            int64_t n_rays, int n_samples, int n_strokes, int shape_width,
                                            int color_width, int color_id, float sdf_delta, int use_laplace_transform,
-                                           int inv_scale_radius, int cone, float coord_div, const float* bg_color,
+                                           int inv_scale_radius, int cone, float coord_div, const float* bg_color, const float* bg_rgbs, int opaque_background,
                                            float cull_eps, int adjoint_exact, const float* grad_rgb,
                                            const float* grad_acc, const float* grad_depth, const float* state,
                                            const int* hit_dir, const int* hit_buf, const float* pair_buf,
@@ -2715,7 +2726,7 @@ N3DS_EXPORT int n3ds_render_batch_backward(const float* origins, const float* di
                      (long long)bs.lds_bytes);
         RenderArgs A = make_args(origins, directions, viewdirs, radii, tdist, table, color_params, density_params,
                                  n_rays, n_samples, sm, shape_width, color_width, color_id, sdf_delta,
-                                 use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, cull_eps,
+                                 use_laplace_transform, inv_scale_radius, cone, coord_div, bg_color, bg_rgbs, opaque_background, cull_eps,
                                  adjoint_exact);
         A.nch_stride = (s_max + 63) / 64;
         A.scenes = scenes;

@@ -15,10 +15,13 @@ stroke, ``strokes.py:95-106``), ``shape_params`` padded to a common width,
 ``color_params`` of one colour type and ``density_params`` (already scaled by
 ``density_scale``).
 """
+import weakref
+
 import torch
 from torch.autograd import Function
 
 from . import _lib
+from .scenes import n_shape_params
 from .strokelib.strokes import get_adjoint
 
 COLOR_WIDTH = [3, 12, 12, 27]
@@ -34,6 +37,26 @@ def prepare_table(stroke_ids, shape_params):
     _lib.check(_lib.lib().n3ds_render_prepare(_lib.ptr(table), _lib.ptr(stroke_ids), _lib.ptr(shape_params), S, Pmax,
                                               _lib.ptr(status), _lib.stream()))
     return table, status
+
+
+# Stroke-id tensors already checked on the device, by id(): (weak reference, version, shape width).
+# render_prep_kernel flags ids it cannot decode (and strokes wider than shape_params); reading that
+# flag costs a host sync, so it is read once per ids tensor and again only after it changes.
+_CHECKED_IDS = {}
+
+
+def _check_device_ids(stroke_ids, Pmax, status):
+    if getattr(stroke_ids, '_n3ds_host_checked', False) or torch.cuda.is_current_stream_capturing():
+        return
+    rec = _CHECKED_IDS.get(id(stroke_ids))
+    if rec is not None and rec[0]() is stroke_ids and rec[1:] == (stroke_ids._version, Pmax):
+        return
+    if int(status.item()):
+        raise RuntimeError('stroke_ids holds an invalid stroke id, or a stroke with more shape parameters than '
+                           f'shape_params has columns ({Pmax})')
+    if len(_CHECKED_IDS) > 256:
+        _CHECKED_IDS.clear()
+    _CHECKED_IDS[id(stroke_ids)] = (weakref.ref(stroke_ids), stroke_ids._version, Pmax)
 
 
 # Kept activations ("store"): per ray, the forward keeps its culled hits and the
@@ -72,7 +95,8 @@ class _render_fn(Function):
     @staticmethod
     def forward(ctx, origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                 density_params, cfg):
-        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store, scenes) = cfg
+        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store, scenes,
+         opaque) = cfg
         R, n1 = tdist.shape
         S, Pmax = shape_params.shape
         if scenes is not None:
@@ -83,8 +107,16 @@ class _render_fn(Function):
             _lib.require_cuda(name, t)
         _lib.require_cuda('stroke_ids', stroke_ids, dtype=torch.int32)
         dev = origins.device
-        table, _ = prepare_table(stroke_ids, shape_params)
-        bgt = (torch.tensor(bg, dtype=torch.float32) if not torch.is_tensor(bg) else bg.float().cpu())
+        table, status = prepare_table(stroke_ids, shape_params)
+        _check_device_ids(stroke_ids, Pmax, status)
+        bg_rays = None
+        if torch.is_tensor(bg) and bg.dim() == 2:  # per-ray background colours [R, 3]
+            bg_rays = bg.to(device=origins.device, dtype=torch.float32).contiguous()
+            if bg_rays.shape != (R, 3):
+                raise RuntimeError(f'bg_color has shape {tuple(bg_rays.shape)}, per-ray colours need ({R}, 3)')
+            bgt = torch.ones(3)
+        else:
+            bgt = torch.as_tensor(bg, dtype=torch.float32).cpu()
         bgv = (_lib.ctypes.c_float * 3)(*[float(v) for v in bgt.reshape(-1).expand(3)])
         rgb = torch.empty(R, 3, dtype=torch.float32, device=dev)
         acc = torch.empty(R, dtype=torch.float32, device=dev)
@@ -108,14 +140,16 @@ class _render_fn(Function):
             _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
             _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax,
             color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
-            _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), float(cull_eps), _lib.ptr(rgb), _lib.ptr(acc),
+            _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), _lib.ptr(bg_rays), int(opaque), float(cull_eps),
+            _lib.ptr(rgb), _lib.ptr(acc),
             _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.ptr(hd), _lib.ptr(hb), _lib.ptr(pb), cap_h,
             cap_p, _lib.ptr(cws), 0 if cws is None else int(cws.numel()), _lib.stream()))
         del cws
         empty = torch.empty(0, device=dev)
         ctx.save_for_backward(origins, directions, viewdirs, radii, tdist, table, shape_params, color_params,
                               density_params, state, hd if hd is not None else empty,
-                              hb if hb is not None else empty, pb if pb is not None else empty)
+                              hb if hb is not None else empty, pb if pb is not None else empty,
+                              bg_rays if bg_rays is not None else empty)
         ctx.caps = (cap_h, cap_p)
         ctx.cfg = cfg
         ctx.bgv = bgv
@@ -124,9 +158,9 @@ class _render_fn(Function):
     @staticmethod
     def backward(ctx, g_rgb, g_acc, g_depth):
         (origins, directions, viewdirs, radii, tdist, table, shape_params, color_params, density_params,
-         state, hd, hb, pb) = ctx.saved_tensors
+         state, hd, hb, pb, bg_rays) = ctx.saved_tensors
         (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store,
-         scenes) = ctx.cfg
+         scenes, opaque) = ctx.cfg
         cap_h, cap_p = ctx.caps
         R, n1 = tdist.shape
         S, Pmax = shape_params.shape
@@ -153,7 +187,8 @@ class _render_fn(Function):
             _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax, Q,
             color_id,
             float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
-            _lib.ctypes.cast(ctx.bgv, _lib.ctypes.c_void_p), float(cull_eps), 1 if adjoint == 'exact' else 0,
+            _lib.ctypes.cast(ctx.bgv, _lib.ctypes.c_void_p), _lib.ptr(bg_rays), int(opaque), float(cull_eps),
+            1 if adjoint == 'exact' else 0,
             _lib.ptr(g_rgb), _lib.ptr(g_acc), _lib.ptr(g_depth), _lib.ptr(state), _lib.ptr(hd), _lib.ptr(hb),
             _lib.ptr(pb), cap_h, cap_p, _lib.ptr(ws), int(ws.numel()), _lib.ptr(gs), _lib.ptr(gc), _lib.ptr(gd),
             _lib.stream()))
@@ -164,7 +199,7 @@ class _render_fn(Function):
 def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                    density_params, color_id=0, sdf_delta=1.0, use_laplace_transform=False, inv_scale_radius=False,
                    cone=False, coord_div=1.0, bg_color=(1.0, 1.0, 1.0), cull_eps=1e-7, adjoint=None,
-                   counters=None, store='auto', scenes=None):
+                   counters=None, store='auto', scenes=None, opaque_background=False):
     """Render rays through a stroke field; returns (rgb [R,3], acc [R], depth [R]).
 
     origins/directions/viewdirs [R,3], radii [R] (sample radius = radii·t if ``cone``
@@ -181,6 +216,14 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
     ``STORE_BUDGET_BYTES``; a ``(cap_hits, cap_pairs)`` tuple sets the per-ray
     capacities.  Results are identical in every mode.
 
+    ``bg_color``: one background colour (3 floats), or per-ray colours
+    ``[R, 3]`` as ``StrokeField`` draws them when ``bg_intensity_range`` is a
+    range (``models.py:213-222``); no gradient flows to it.
+    ``opaque_background``: the last interval is infinitely wide
+    (``render.compute_alpha_weights``, ``utils/render.py:35-40``).
+    ``inv_scale_radius`` defaults to False here (the 500-stroke benchmark's
+    setting); ``StrokeField`` itself defaults to True (``models.py:540``).
+
     ``scenes``: a ``multiscene.SceneBatch`` when the arrays hold many
     independent scenes packed one after another (each ray composites only its
     own scene's strokes); every stage then runs as one launch for all scenes.
@@ -190,13 +233,18 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
                            f'{COLOR_WIDTH[color_id]}')
     if stroke_ids.device.type == 'cpu':
         ids = stroke_ids.tolist()
-        for i in ids:
+        for k, i in enumerate(ids):
             base, flags = i >> 4, i & 15
             if base >= N_BASE or (base >= 9 and flags) or ((flags & 4) and (flags & 8)):
                 raise RuntimeError(f'invalid stroke id {i}')
+            if n_shape_params(i) > shape_params.shape[1]:
+                raise RuntimeError(f'stroke {k} (id {i}) has {n_shape_params(i)} shape parameters, shape_params '
+                                   f'has width {shape_params.shape[1]}')
         stroke_ids = stroke_ids.to(device=shape_params.device, dtype=torch.int32)
+        stroke_ids._n3ds_host_checked = True
     cfg = (int(color_id), float(sdf_delta), bool(use_laplace_transform), bool(inv_scale_radius), bool(cone),
-           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters, store, scenes)
+           float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters, store, scenes,
+           bool(opaque_background))
     f = lambda t: t.contiguous().float()
     return _render_fn.apply(f(origins), f(directions), f(viewdirs), f(radii), f(tdist), stroke_ids.contiguous(),
                             f(shape_params), f(color_params), f(density_params), cfg)

@@ -33,14 +33,14 @@ def _forward(sc, dev):
     state = torch.empty(5, R * (n1 - 1), device=dev)
     _lib.check(L.n3ds_render_forward(
         _lib.ptr(o), _lib.ptr(d), _lib.ptr(vd), _lib.ptr(rad), _lib.ptr(td), _lib.ptr(table), _lib.ptr(cp),
-        _lib.ptr(dp), R, n1 - 1, S, P, Q, 0, 1.0, 0, 0, 0, 1.0, ctypes.cast(bg, ctypes.c_void_p), 1e-7,
+        _lib.ptr(dp), R, n1 - 1, S, P, Q, 0, 1.0, 0, 0, 0, 1.0, ctypes.cast(bg, ctypes.c_void_p), None, 0, 1e-7,
         _lib.ptr(rgb), _lib.ptr(acc), _lib.ptr(depth), _lib.ptr(state), None, None, None, None, 0, 0, None, 0,
         _lib.stream()))
     g = torch.Generator().manual_seed(S)
     g_rgb = torch.randn(R, 3, generator=g).to(dev)
     g_acc = torch.randn(R, generator=g).to(dev)
     common = (_lib.ptr(o), _lib.ptr(d), _lib.ptr(vd), _lib.ptr(rad), _lib.ptr(td), _lib.ptr(table), _lib.ptr(cp),
-              _lib.ptr(dp), R, n1 - 1, S, P, Q, 0, 1.0, 0, 0, 0, 1.0, ctypes.cast(bg, ctypes.c_void_p), 1e-7, 1,
+              _lib.ptr(dp), R, n1 - 1, S, P, Q, 0, 1.0, 0, 0, 0, 1.0, ctypes.cast(bg, ctypes.c_void_p), None, 0, 1e-7, 1,
               _lib.ptr(g_rgb), _lib.ptr(g_acc), None, _lib.ptr(state), None, None, None, 0, 0)
     keep = (o, d, vd, rad, td, sp, cp, dp, table, state, g_rgb, g_acc, bg)
     return common, (S, P, Q, R), keep
@@ -95,7 +95,7 @@ def test_no_alloc_backward_rejects_oversized_stroke_sets(dev):
     bg = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
     z = torch.zeros(4, device=dev)
     st = L.n3ds_render_backward_noalloc(*([_lib.ptr(z)] * 8), 1, 64, S, 14, 3, 0, 1.0, 0, 0, 0, 1.0,
-                                        ctypes.cast(bg, ctypes.c_void_p), 1e-7, 1, _lib.ptr(z), None, None,
+                                        ctypes.cast(bg, ctypes.c_void_p), None, 0, 1e-7, 1, _lib.ptr(z), None, None,
                                         _lib.ptr(z), None, None, None, 0, 0, _lib.ptr(z), _lib.ptr(z), _lib.ptr(z),
                                         _lib.stream())
     assert st == 1

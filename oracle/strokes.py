@@ -10,7 +10,8 @@ Follows, in order of the reference's own files:
   (restated in :func:`grad_sdf_reference`, including their known inexact terms);
 * colour fields ``strokes_color.h:34-171`` and SH basis ``sh.h:8-30``;
 * per-(point, stroke) forward ``strokes_forward.cu:35-109``;
-* per-(point, stroke) adjoint ``strokes_backward.cu:38-157``.
+* per-(point, stroke) adjoint ``strokes_backward.cu:38-157``;
+* texture coordinates ``BaseSDF<...>::texcoord`` (``strokes_sdf.h:258-526``).
 
 Two adjoints are provided:
 
@@ -29,6 +30,12 @@ values at axis points, capsule on its axis), brute-force Bezier distances by
 dense parameter sampling, rotation orthogonality and angle conventions,
 finite differences of the forward in float64 for every exact adjoint, and
 hand-derived derivatives for every reference-adjoint term that is exact.
+``tests/test_oracle_reference_surrogates.py`` rebuilds every reference-mode
+adjoint, the inexact terms included, as torch.autograd of a surrogate forward
+(the real values, the reference's local derivatives), so the hand chaining
+here is checked by an independent route; ``tests/test_oracle_texcoord.py``
+pins the texture coordinates by closed-form points, invariances and a sort of
+the scale axes.
 """
 import math
 
@@ -541,6 +548,69 @@ BASE_SDF_FNS = [sdf_sphere, sdf_cube, sdf_round_cube, sdf_capped_torus, sdf_caps
 
 def base_sdf(base, p, prm):
     return BASE_SDF_FNS[base](p, prm)
+
+
+# ----------------------------------------------------------------------------
+# texture coordinates (BaseSDF<...>::texcoord, written by strokes_forward.cu:105-107)
+# ----------------------------------------------------------------------------
+
+def permute_multiscale_axis(d, scale):
+    """strokes_sdf.h:211-222: the components reordered by decreasing scale (component lists)."""
+    sx, sy, sz = scale
+    xy, yz, xz = sx > sy, sy > sz, sx > sz
+    zy = sz > sy
+    first = [d[0], d[1], d[2]], [d[0], d[2], d[1]], [d[2], d[0], d[1]]
+    second = [d[2], d[1], d[0]], [d[1], d[0], d[2]], [d[1], d[2], d[0]]
+    out = []
+    for i in range(3):
+        when_xy = torch.where(yz, first[0][i], torch.where(xz, first[1][i], first[2][i]))
+        when_not = torch.where(zy, second[0][i], torch.where(xz, second[1][i], second[2][i]))
+        out.append(torch.where(xy, when_xy, when_not))
+    return out
+
+
+def texcoord(base, p, row, multiscale):
+    """(u, v) of a unit-space point p (component list) for a stroke whose parameter row is ``row``
+    ([..., P], the whole row as the reference's ``params`` pointer); (0, 0) for the bases without
+    texture coordinates.  Sphere strokes_sdf.h:258-268, box / round box :301-311 / :357-367
+    (multiscale permutation from params + 0), capsule :456-469 and line :511-526 (swap of x and
+    z when scale_z > scale_x, scale read from params + 1)."""
+    zero = torch.zeros_like(p[0])
+    if base == 0:
+        n = torch.rsqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2])
+        d = [p[i] * n for i in range(3)]
+        if multiscale:
+            d = permute_multiscale_axis(d, [row[..., i] for i in range(3)])
+        u = torch.asin(torch.clamp(d[0], -1.0, 1.0)) / math.pi + 0.5
+        v = torch.atan2(d[2], d[1]) / (2.0 * math.pi) + 0.5
+        return u, v
+    if base in (1, 2):
+        uvt = [p[i] * 0.5 + 0.5 for i in range(3)]
+        if multiscale:
+            uvt = permute_multiscale_axis(uvt, [row[..., i] for i in range(3)])
+        return torch.clamp(uvt[0], 0.0, 1.0), torch.clamp(uvt[1], 0.0, 1.0)
+    if base in (4, 5):
+        h = row[..., 0]
+        if base == 4:
+            u = torch.clamp((p[1] / (h + 1.0)) * 0.5 + 0.5, 0.0, 1.0)
+        else:
+            u = torch.clamp((p[1] + h + 1.0 - row[..., 1]) / (2.0 * h + 2.0), 0.0, 1.0)
+        px, pz = p[0], p[2]
+        if multiscale:
+            swap = row[..., 3] > row[..., 1]
+            px, pz = torch.where(swap, p[2], p[0]), torch.where(swap, p[0], p[2])
+        v = torch.atan2(pz, px) / (2.0 * math.pi) + 0.5
+        return u, v
+    return zero, zero
+
+
+def stroke_texcoords(x, shape_params, sdf_id):
+    """texcoord [N, S, 2] of points x [N, 3] for strokes [S, P] of one type."""
+    base, _, _, _, ms = decode_sdf_id(sdf_id)
+    p = inverse_transform(x[:, None, :], shape_params[None], sdf_id)
+    p = [pi.expand(x.shape[0], shape_params.shape[0]) for pi in p]
+    u, v = texcoord(base, p, shape_params[None].expand(x.shape[0], -1, -1), ms)
+    return torch.stack([u, v], -1)
 
 
 # ----------------------------------------------------------------------------

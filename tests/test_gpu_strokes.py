@@ -148,3 +148,30 @@ def test_stroke_errors(dev):
         fn(x.to(dev), r.to(dev), vd.to(dev), sp[:, :5].contiguous().to(dev), cp.to(dev), 1.0)
     with pytest.raises(RuntimeError):  # CPU tensors are rejected like TORCH_CHECK(is_cuda)
         fn(x, r, vd, sp, cp, 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('shape', SHAPES)
+def test_stroke_texcoords(shape, dev):
+    """return_texcoord=True against the oracle's texcoord (strokes_sdf.h:258-526, written by
+    strokes_forward.cu:105-107); bases without coordinates return (0, 0).  Pairs within 1e-5 of
+    the azimuth's branch cut (v jumps from 1 to 0 there) are left out."""
+    color = 'constant_rgb'
+    sp, cp, x, r, vd = _setup(shape, color, 2)
+    fn = dstrokes.get_stroke(shape, color, 'recon')[0]
+    _, _, _, tex = fn(x.to(dev), r.to(dev), vd.to(dev), sp.to(dev), cp.to(dev), 1.0, False, False, True, True)
+    sdf_id = ostrokes.shape_sdf_id(shape)
+    S = sp.shape[0]
+    ref = ostrokes.stroke_texcoords(x.reshape(-1, 3).double(), sp.double(), sdf_id)
+    got = tex.reshape(-1, S, 2).double().cpu()
+    base = sdf_id >> 4
+    # the azimuth's branch cut is where v is 0 or 1
+    keep = (ref[..., 1] > 1e-5) & (ref[..., 1] < 1 - 1e-5) if base in (0, 4, 5) else torch.ones_like(ref[..., 0],
+                                                                                                      dtype=torch.bool)
+    assert keep.float().mean() > 0.99
+    err = (got - ref).abs()[keep]
+    assert float(err.max()) < 2e-5, float(err.max())
+    if base in (0, 1, 2, 4, 5):
+        assert float(ref[..., 0].std()) > 0.05  # the points spread over the coordinates
+    else:
+        assert float(got.abs().max()) == 0.0

@@ -642,6 +642,8 @@ def main():
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
     ap.add_argument('--autotune', type=int, default=1, help='tune launch shapes on the workload first (0: defaults)')
+    ap.add_argument('--tune-set', default='', help='launch shapes to set instead of tuning, e.g. '
+                                                   'cull_group=16,flush_pairs=256 (profiler runs)')
     ap.add_argument('--copy-calibration', type=int, default=0,
                     help='only copy a device buffer of this many MiB 5 times (HBM counter calibration run)')
     args = ap.parse_args()
@@ -696,7 +698,13 @@ def main():
     torch.cuda.synchronize()
     # launch shapes chosen by the autotuner on this workload (untimed)
     tuning, tuning_ms = None, None
-    if args.autotune:
+    if args.tune_set:
+        from neural3dstrokes_amd import autotune
+        for kv in args.tune_set.split(','):
+            k, v = kv.split('=')
+            autotune.set(k.strip(), int(v))
+        tuning = autotune.current()
+    elif args.autotune:
         from neural3dstrokes_amd import autotune
         tuning, tuning_ms = autotune.tune(fused, names=('cull_group', 'flush_pairs', 'bwd_oversub'))
     t0 = torch.cuda.Event(enable_timing=True)

@@ -429,6 +429,72 @@ def stepfun_block(seed=0, n_rays=65536, n_bins=64, n_samples=128, steps=20):
             'max_abs_err_vs_fp64': float(np.abs(got - want).max())}
 
 
+def prop_block(n_points=65536 * 64, steps=10, warmup=3, seed=0):
+    """Fused proposal density field (PropMLP defaults: 10 levels x 1 feature, 2^21 rows, resolution
+    16 -> 8192, Linear(10, 64) -> ReLU -> Linear(64, 1), softplus) fwd+bwd on the sample positions
+    of 65536 rays x 64 proposal samples, against the plain torch formulation (warp as torch ops,
+    GridEncoder, nn.Linear, softplus, autograd) on the same GPU."""
+    import torch
+    from neural3dstrokes_amd import propfield
+    dev = torch.device('cuda:0')
+    torch.manual_seed(seed)
+    m = propfield.PropDensityField().to(dev).eval()
+    with torch.no_grad():
+        m.encoder.embeddings.uniform_(-0.3, 0.3)
+    g = torch.Generator().manual_seed(seed + 1)
+    coords = (torch.randn(n_points, 3, generator=g) * 2.0).to(dev)
+    gd = torch.randn(n_points, generator=g).to(dev)
+    params = list(m.parameters())
+
+    def fused():
+        for p in params:
+            p.grad = None
+        m(coords)['density'].backward(gd)
+
+    def plain():
+        for p in params:
+            p.grad = None
+        propfield.reference_forward(m, coords).backward(gd)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def fused_fwd():
+        with torch.no_grad():
+            m(coords)
+
+    def plain_fwd():
+        with torch.no_grad():
+            propfield.reference_forward(m, coords)
+
+    t_f, t_p = timed(fused), timed(plain)
+    t_ff, t_pf = timed(fused_fwd), timed(plain_fwd)
+    fused()
+    g_f = [p.grad.clone() for p in params]
+    plain()
+    g_p = [p.grad.clone() for p in params]
+    Fd = m.encoder.output_dim
+    flops = n_points * (2 * 64 * Fd + 2 * 64 + 3 * (2 * 64 * Fd) + 4 * 64)
+    return {'workload': f'{n_points} points (65536 rays x 64 proposal samples), PropMLP defaults: '
+                        f'{m.encoder.num_levels} levels x {m.encoder.level_dim}, 2^21 rows, res 16->8192, '
+                        f'Linear({Fd},64)-ReLU-Linear(64,1)-softplus, contract warp',
+            'fused_fwd_bwd_ms': t_f, 'plain_torch_fwd_bwd_ms': t_p, 'speedup_vs_plain': t_p / t_f,
+            'fused_fwd_ms': t_ff, 'plain_torch_fwd_ms': t_pf,
+            'points_per_s': n_points / (t_f * 1e-3),
+            'mlp_gflop_per_step': flops / 1e9, 'mlp_tflops': flops / (t_f * 1e-3) / 1e12,
+            'rel_err_fused_vs_plain_grads': [float((a - b).norm() / b.norm().clamp_min(1e-30))
+                                             for a, b in zip(g_f, g_p)]}
+
+
 def memory_block(steps=3, warmup=1):
     """640,000 rays x 192 samples x 1000 strokes: fused fwd+bwd with the forward's hits / alpha
     kept for the adjoint versus full recomputation; throughput and peak HBM for both."""
@@ -640,6 +706,7 @@ def main():
     ap.add_argument('--texture', type=int, default=1, help='image-texture block (0: skip)')
     ap.add_argument('--stepfun', type=int, default=1, help='proposal interval-resampling block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
+    ap.add_argument('--prop', type=int, default=1, help='fused proposal density field block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
     ap.add_argument('--autotune', type=int, default=1, help='tune launch shapes on the workload first (0: defaults)')
     ap.add_argument('--tune-set', default='', help='launch shapes to set instead of tuning, e.g. '
@@ -806,6 +873,8 @@ def main():
         out['interval_resampling'] = stepfun_block()
     if args.memory:
         out['memory_bounded'] = memory_block()
+    if args.prop:
+        out['proposal_field'] = prop_block()
     if args.multiscene:
         out['multiscene'] = multiscene_block()
     if args.cpu_rays > 0:

@@ -127,6 +127,38 @@ int n3ds_grid_encode_backward(const void* grad, const float* inputs, const int32
  * backward scatters with atomics only). */
 int64_t n3ds_grid_encode_backward_workspace_bytes(int64_t B, int D, int C, int L, int64_t n_rows);
 
+/* Fused proposal density field: the forward of the reference's PropMLP with
+ * disable_rgb (source/models.py:286-523: _warp_coords :19-33, GridEncoder with
+ * bound 1, density_layer Linear(F, 64) -> ReLU -> Linear(64, 1), softplus of
+ * raw + density_bias), F = num_levels * level_dim <= 32, in one kernel.
+ * coords [N,3]; warp 0: none (coords already in [-1, 1]), 1: coord.contract
+ * then / 2 (the 'contract' warp), 2: divide by box_size / 2; the table as for
+ * n3ds_grid_encode_forward (S = log2(per_level_scale), H = base resolution,
+ * dtype 0 float32 / 1 float16); w1 [64, F], b1 [64], w2 [1, 64], b2 [1] float32
+ * (nn.Linear's layout).  density_out [N]; coord_out [N,3] (the warped
+ * coordinates the reference returns as `coord`, or NULL); features_out [N, F]
+ * and grid_inputs_out [N,3] (the encoder's outputs and inputs, kept for the
+ * adjoint; both or neither, NULL when no gradient is needed). */
+int n3ds_prop_field_forward(const float* coords, int64_t n_points, int warp, float box_size, const void* embeddings,
+                            const int32_t* offsets, int num_levels, int level_dim, float S, int H, int gridtype,
+                            int align_corners, int interp, int dtype, const float* w1, const float* b1,
+                            const float* w2, const float* b2, float density_bias, float* density_out,
+                            float* coord_out, float* features_out, float* grid_inputs_out, void* stream);
+
+/* Adjoint of n3ds_prop_field_forward for upstream grad_density [N]: reads the
+ * kept features [N, F] (or, when NULL, gathers them again from the table and
+ * writes grid_inputs [N,3]), recomputes the hidden units per point (the [N, 64]
+ * activations are never stored), writes grad_features [N, F] (dL/d of the
+ * encoder outputs) for n3ds_grid_encode_backward, and adds the weight gradients
+ * into grad_w1 [64,F], grad_b1 [64], grad_w2 [64], grad_b2 [1]
+ * (zero-initialised by the caller). */
+int n3ds_prop_field_backward(const float* coords, int64_t n_points, int warp, float box_size, const void* embeddings,
+                             const int32_t* offsets, int num_levels, int level_dim, float S, int H, int gridtype,
+                             int align_corners, int interp, int dtype, const float* w1, const float* b1,
+                             const float* w2, const float* b2, float density_bias, const float* grad_density,
+                             const float* features, float* grad_features, float* grid_inputs, float* grad_w1,
+                             float* grad_b1, float* grad_w2, float* grad_b2, void* stream);
+
 /* Replaces the pybind `grad_total_variation` of gridencoder.h:15
  * (kernel_grad_tv gridencoder.cu:506-645), called from grid.py:172-194.
  * Adds weight * TV gradient into grad_f32 [n_rows,C] float32. */

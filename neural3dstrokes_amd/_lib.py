@@ -35,6 +35,10 @@ SIGNATURES = {
                                          _I32, _I32, _P, _I64, _P]),
     'n3ds_grid_encode_backward_workspace_bytes': (_I64, [_I64, _I32, _I32, _I32, _I64]),
     'n3ds_grid_grad_tv': (_I32, [_P, _P, _P, _P, _F, _I64, _I32, _I32, _I32, _F, _I32, _I32, _I32, _I32, _P]),
+    'n3ds_prop_field_forward': (_I32, [_P, _I64, _I32, _F, _P, _P, _I32, _I32, _F, _I32, _I32, _I32, _I32, _I32, _P,
+                                       _P, _P, _P, _F, _P, _P, _P, _P, _P]),
+    'n3ds_prop_field_backward': (_I32, [_P, _I64, _I32, _F, _P, _P, _I32, _I32, _F, _I32, _I32, _I32, _I32, _I32, _P,
+                                        _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'n3ds_grad_sqnorm_accumulate': (_I32, [_P, _I64, _P, _P]),
     'n3ds_grad_clip_coef': (_I32, [_P, _F, _P, _P]),
     'n3ds_adam_step': (_I32, [_P, _P, _P, _P, _I64, _I32, _P, _P, _D, _P, _D, _D, _D, _I64, _P, _F, _P, _F, _F, _P]),

@@ -633,6 +633,251 @@ __global__ __launch_bounds__(GB_THREADS) void grid_bucket_reduce_kernel(GridBuck
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused proposal density field: the reference's PropMLP forward
+// (source/models.py:286-523 with disable_rgb: _warp_coords, GridEncoder with
+// bound 1, density_layer = Linear(F, 64) -> ReLU -> Linear(64, 1), then
+// softplus(raw + density_bias)), F = L * C grid features, in one kernel; the
+// adjoint recomputes features and hidden units per point instead of storing
+// the [N, 64] activations, writes dL/dfeatures for the grid scatter and
+// reduces the weight gradients in LDS.
+// ---------------------------------------------------------------------------
+constexpr int PF_HID = 64;
+constexpr int PF_MAXF = 32;
+constexpr int PF_FWD_THREADS = 256;
+constexpr int PF_BWD_THREADS = 128;
+constexpr int PF_CH = 16;  // hidden units per weight-gradient chunk
+
+struct PropArgs {
+    const float* coords;  // [N, 3] sample positions
+    int64_t N;
+    int warp;             // 0: none (coords in [-1, 1]), 1: contract then / 2, 2: divide by half the box side
+    float inv_half_box;
+    const void* emb;
+    const int* offsets;
+    int L;
+    LevelTab lt;
+    int gridtype, align, interp;
+    const float* W1;  // [64, F]
+    const float* b1;  // [64]
+    const float* W2;  // [1, 64]
+    const float* b2;  // [1]
+    float dbias;
+};
+
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// models.py:19-33 and coord.contract (utils/coord.py:5-11), then grid.py:158
+// (inputs + bound) / (2 bound) with bound 1
+__device__ __forceinline__ void prop_warp(const PropArgs& P, int64_t b, float* cw, float* x01) {
+    float x[3] = {P.coords[b * 3], P.coords[b * 3 + 1], P.coords[b * 3 + 2]};
+    if (P.warp == 1) {
+        const float eps = 1.1920929e-07f;
+        const float m2 = fmaxf(x[0] * x[0] + x[1] * x[1] + x[2] * x[2], eps);
+        const float k = m2 <= 1.f ? 1.f : (2.f * sqrtf(m2) - 1.f) / m2;
+#pragma unroll
+        for (int d = 0; d < 3; d++) x[d] = (m2 <= 1.f ? x[d] : k * x[d]) / 2.f;
+    } else if (P.warp == 2) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) x[d] = x[d] * P.inv_half_box;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        cw[d] = x[d];
+        x01[d] = (x[d] + 1.f) / 2.f;
+    }
+}
+
+template <typename T, int C, int FM>
+__device__ __forceinline__ void prop_features(const PropArgs& P, const float* x01, float* f) {
+#pragma unroll
+    for (int l = 0; l < FM / C; l++) {
+        float a[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) a[c] = 0.f;
+        if (l < P.L) grid_level_acc<T, 3, C>(x01, l, P.offsets, P.lt, (const T*)P.emb, P.gridtype, P.align, P.interp, a);
+#pragma unroll
+        for (int c = 0; c < C; c++) f[l * C + c] = a[c];
+    }
+}
+
+// torch.nn.functional.softplus (beta 1, threshold 20) and its derivative
+__device__ __forceinline__ float softplus20(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+__device__ __forceinline__ float dsoftplus20(float x) { return x > 20.f ? 1.f : 1.f / (1.f + expf(-x)); }
+
+template <int FM>
+__device__ __forceinline__ float prop_raw(const float* sW1, const float* sb1, const float* sW2, float b2, int F,
+                                          const float* f) {
+    float raw = b2;
+#pragma unroll 4
+    for (int j = 0; j < PF_HID; j++) {
+        float a = sb1[j];
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+            if (i < F) a += sW1[j * F + i] * f[i];
+        raw += sW2[j] * fmaxf(a, 0.f);
+    }
+    return raw;
+}
+
+// FM: per-lane feature capacity (16 or 32 >= F)
+// FX > 0: the feature count is FX (= FM) at compile time
+template <typename T, int C, int FM, int FX = 0>
+__global__ __launch_bounds__(PF_FWD_THREADS) void prop_fwd_kernel(PropArgs P, float* __restrict__ density,
+                                                                  float* __restrict__ coord_out,
+                                                                  float* __restrict__ feat_out,
+                                                                  float* __restrict__ x01_out) {
+    __shared__ float sW1[PF_HID * PF_MAXF], sb1[PF_HID], sW2[PF_HID];
+    const int F = FX > 0 ? FX : P.L * C;
+    for (int i = threadIdx.x; i < PF_HID * F; i += PF_FWD_THREADS) sW1[i] = P.W1[i];
+    for (int i = threadIdx.x; i < PF_HID; i += PF_FWD_THREADS) {
+        sb1[i] = P.b1[i];
+        sW2[i] = P.W2[i];
+    }
+    __syncthreads();
+    const float b2 = P.b2[0];
+    for (int64_t b = (int64_t)blockIdx.x * PF_FWD_THREADS + threadIdx.x; b < P.N;
+         b += (int64_t)gridDim.x * PF_FWD_THREADS) {
+        float cw[3], x01[3], f[FM];
+        prop_warp(P, b, cw, x01);
+        prop_features<T, C, FM>(P, x01, f);
+        const float raw = prop_raw<FM>(sW1, sb1, sW2, b2, F, f);
+        density[b] = softplus20(raw + P.dbias);
+        if (coord_out) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) coord_out[b * 3 + d] = cw[d];
+        }
+        if (feat_out) {  // kept for the adjoint, which then gathers nothing
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+                if (i < F) feat_out[b * F + i] = f[i];
+#pragma unroll
+            for (int d = 0; d < 3; d++) x01_out[b * 3 + d] = x01[d];
+        }
+    }
+}
+
+template <typename T, int C, int FM, int FX = 0>
+__global__ __launch_bounds__(PF_BWD_THREADS) void prop_bwd_kernel(PropArgs P, const float* __restrict__ gdens,
+                                                                  const float* __restrict__ feat_in,
+                                                                  float* __restrict__ gfeat, float* __restrict__ x01_out,
+                                                                  float* __restrict__ gW1, float* __restrict__ gb1,
+                                                                  float* __restrict__ gW2, float* __restrict__ gb2) {
+    constexpr int NW = PF_BWD_THREADS / 64;
+    __shared__ float sW1[PF_HID * PF_MAXF], sb1[PF_HID], sW2[PF_HID];
+    __shared__ float aW1[PF_HID * PF_MAXF], ab1[PF_HID], aW2[PF_HID], ab2[1];  // block accumulators
+    __shared__ float sf[NW][64][FM + 1];                                        // features, point-major
+    __shared__ float sdh[NW][64][PF_CH + 1], sh[NW][64][PF_CH + 1];            // one chunk of hidden units
+    __shared__ float sg[NW][64];
+    const int F = FX > 0 ? FX : P.L * C;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < PF_HID * F; i += PF_BWD_THREADS) {
+        sW1[i] = P.W1[i];
+        aW1[i] = 0.f;
+    }
+    for (int i = threadIdx.x; i < PF_HID; i += PF_BWD_THREADS) {
+        sb1[i] = P.b1[i];
+        sW2[i] = P.W2[i];
+        ab1[i] = 0.f;
+        aW2[i] = 0.f;
+    }
+    if (threadIdx.x == 0) ab2[0] = 0.f;
+    __syncthreads();
+    const float b2 = P.b2[0];
+    // every lane runs the same number of tiles (points past N contribute zeros)
+    const int64_t ntile = (P.N + PF_BWD_THREADS - 1) / PF_BWD_THREADS;
+    float gb2_lane = 0.f;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t b = tile * PF_BWD_THREADS + threadIdx.x;
+        const bool live = b < P.N;
+        float f[FM], df[FM], g = 0.f;
+#pragma unroll
+        for (int i = 0; i < FM; i++) { f[i] = 0.f; df[i] = 0.f; }
+        if (live) {
+            if (feat_in) {
+#pragma unroll
+                for (int i = 0; i < FM; i++)
+                    if (i < F) f[i] = feat_in[b * F + i];
+            } else {
+                float cw[3], x01[3];
+                prop_warp(P, b, cw, x01);
+                prop_features<T, C, FM>(P, x01, f);
+#pragma unroll
+                for (int d = 0; d < 3; d++) x01_out[b * 3 + d] = x01[d];
+            }
+            const float raw = prop_raw<FM>(sW1, sb1, sW2, b2, F, f);
+            g = gdens[b] * dsoftplus20(raw + P.dbias);
+        }
+        gb2_lane += g;
+#pragma unroll
+        for (int i = 0; i < FM; i++) sf[wave][lane][i] = f[i];
+        sg[wave][lane] = g;
+        for (int j0 = 0; j0 < PF_HID; j0 += PF_CH) {
+            // per point: the chunk's hidden units, their adjoints and dL/df
+#pragma unroll 4
+            for (int jj = 0; jj < PF_CH; jj++) {
+                const int j = j0 + jj;
+                float a = sb1[j];
+#pragma unroll
+                for (int i = 0; i < FM; i++)
+                    if (i < F) a += sW1[j * F + i] * f[i];
+                const float dh = a > 0.f ? g * sW2[j] : 0.f;
+#pragma unroll
+                for (int i = 0; i < FM; i++)
+                    if (i < F) df[i] += dh * sW1[j * F + i];
+                sdh[wave][lane][jj] = dh;
+                sh[wave][lane][jj] = fmaxf(a, 0.f);
+            }
+            wave_fence();
+            // per hidden unit: sums over the wave's 64 points; lane = (unit jj, feature group ig)
+            constexpr int KG = FM / 4;  // features per lane group
+            const int jj = lane & (PF_CH - 1), ig = lane >> 4;  // 4 groups of KG features
+            float w1[KG], bsum = 0.f, w2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < KG; k++) w1[k] = 0.f;
+            for (int q = 0; q < 64; q++) {
+                const float dh = sdh[wave][q][jj];
+#pragma unroll
+                for (int k = 0; k < KG; k++) w1[k] += dh * sf[wave][q][ig * KG + k];
+                if (ig == 0) {
+                    bsum += dh;
+                    w2 += sg[wave][q] * sh[wave][q][jj];
+                }
+            }
+            const int j = j0 + jj;
+#pragma unroll
+            for (int k = 0; k < KG; k++)
+                if (ig * KG + k < F) atomicAdd(&aW1[j * F + ig * KG + k], w1[k]);
+            if (ig == 0) {
+                atomicAdd(&ab1[j], bsum);
+                atomicAdd(&aW2[j], w2);
+            }
+            wave_fence();
+        }
+        if (live) {
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+                if (i < F) gfeat[b * F + i] = df[i];
+        }
+    }
+    // db2 = sum of g over all points
+    float v = gb2_lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) atomicAdd(&ab2[0], v);
+    __syncthreads();
+    for (int i = threadIdx.x; i < PF_HID * F; i += PF_BWD_THREADS) atomicAdd(&gW1[i], aW1[i]);
+    for (int i = threadIdx.x; i < PF_HID; i += PF_BWD_THREADS) {
+        atomicAdd(&gb1[i], ab1[i]);
+        atomicAdd(&gW2[i], aW2[i]);
+    }
+    if (threadIdx.x == 0) atomicAdd(&gb2[0], ab2[0]);
+}
+
 #define N3DS_GRID_DISPATCH(KERNEL_LAUNCH)                                                           \
     switch (D * 16 + C) {                                                                          \
         case 2 * 16 + 1: KERNEL_LAUNCH(2, 1); break;                                               \
@@ -802,6 +1047,128 @@ N3DS_EXPORT int n3ds_grid_grad_tv(const float* inputs, const void* embeddings, f
                            align_corners);
     N3DS_GRID_DISPATCH(LAUNCH_T)
 #undef LAUNCH_T
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused proposal density field entry points
+// ---------------------------------------------------------------------------
+static int prop_check(int64_t N, int C, int L, int dtype, int warp) {
+    N3DS_REQUIRE(N >= 0, "bad point count");
+    N3DS_REQUIRE(C == 1 || C == 2 || C == 4, "level_dim must be 1, 2 or 4 (got %d)", C);
+    N3DS_REQUIRE(L >= 1 && L * C <= PF_MAXF, "num_levels * level_dim must be in [1, %d] (got %d x %d)", PF_MAXF, L, C);
+    N3DS_REQUIRE(dtype == 0 || dtype == 1, "dtype must be 0 (float32) or 1 (float16)");
+    N3DS_REQUIRE(warp >= 0 && warp <= 2, "warp must be 0 (none), 1 (contract) or 2 (box)");
+    return N3DS_OK;
+}
+
+static int g_pf_cu = 0;
+
+static int prop_blocks(int64_t work, int threads, int per_cu, int64_t* out) {
+    if (g_pf_cu == 0) {
+        int dev;
+        N3DS_HIP_CHECK(hipGetDevice(&dev));
+        N3DS_HIP_CHECK(hipDeviceGetAttribute(&g_pf_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    int64_t nb = (work + threads - 1) / threads;
+    const int64_t cap = (int64_t)g_pf_cu * per_cu;
+    *out = nb < 1 ? 1 : (nb > cap ? cap : nb);
+    return N3DS_OK;
+}
+
+static PropArgs prop_args(const float* coords, int64_t N, int warp, float box_size, const void* emb,
+                          const int32_t* offsets, int L, float S, int H, int gridtype, int align, int interp,
+                          const float* W1, const float* b1, const float* W2, const float* b2, float dbias) {
+    PropArgs P;
+    P.coords = coords; P.N = N; P.warp = warp; P.inv_half_box = box_size > 0.f ? 2.f / box_size : 1.f;
+    P.emb = emb; P.offsets = offsets; P.L = L; P.lt = make_level_tab(L, S, H);
+    P.gridtype = gridtype; P.align = align; P.interp = interp;
+    P.W1 = W1; P.b1 = b1; P.W2 = W2; P.b2 = b2; P.dbias = dbias;
+    return P;
+}
+
+N3DS_EXPORT int n3ds_prop_field_forward(const float* coords, int64_t n_points, int warp, float box_size,
+                                        const void* embeddings, const int32_t* offsets, int num_levels,
+                                        int level_dim, float S, int H, int gridtype, int align_corners, int interp,
+                                        int dtype, const float* w1, const float* b1, const float* w2,
+                                        const float* b2, float density_bias, float* density_out,
+                                        float* coord_out, float* features_out, float* grid_inputs_out,
+                                        void* stream) {
+    N3DS_REQUIRE((features_out == nullptr) == (grid_inputs_out == nullptr),
+                 "features_out and grid_inputs_out are kept together");
+    int st = prop_check(n_points, level_dim, num_levels, dtype, warp);
+    if (st) return st;
+    if (n_points == 0) return N3DS_OK;
+    const PropArgs P = prop_args(coords, n_points, warp, box_size, embeddings, offsets, num_levels, S, H, gridtype,
+                                 align_corners, interp, w1, b1, w2, b2, density_bias);
+    int64_t nb;
+    st = prop_blocks(n_points, PF_FWD_THREADS, 8, &nb);
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+#define LAUNCH_PF(CC, FM, FX)                                                                                    \
+    if (dtype == 0)                                                                                              \
+        hipLaunchKernelGGL((prop_fwd_kernel<float, CC, FM, FX>), dim3((unsigned)nb), dim3(PF_FWD_THREADS), 0, s, \
+                           P, density_out, coord_out, features_out, grid_inputs_out);                            \
+    else                                                                                                         \
+        hipLaunchKernelGGL((prop_fwd_kernel<__half, CC, FM, FX>), dim3((unsigned)nb), dim3(PF_FWD_THREADS), 0,   \
+                           s, P, density_out, coord_out, features_out, grid_inputs_out);
+    const bool small = num_levels * level_dim <= 16;
+    if (level_dim == 1 && num_levels == 10) {
+        LAUNCH_PF(1, 10, 10)
+    } else if (level_dim == 2 && num_levels == 10) {
+        LAUNCH_PF(2, 20, 20)
+    } else if (level_dim == 1) {
+        if (small) { LAUNCH_PF(1, 16, 0) } else { LAUNCH_PF(1, 32, 0) }
+    } else if (level_dim == 2) {
+        if (small) { LAUNCH_PF(2, 16, 0) } else { LAUNCH_PF(2, 32, 0) }
+    } else {
+        if (small) { LAUNCH_PF(4, 16, 0) } else { LAUNCH_PF(4, 32, 0) }
+    }
+#undef LAUNCH_PF
+    N3DS_LAUNCH_CHECK();
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int n3ds_prop_field_backward(const float* coords, int64_t n_points, int warp, float box_size,
+                                         const void* embeddings, const int32_t* offsets, int num_levels,
+                                         int level_dim, float S, int H, int gridtype, int align_corners, int interp,
+                                         int dtype, const float* w1, const float* b1, const float* w2,
+                                         const float* b2, float density_bias, const float* grad_density,
+                                         const float* features, float* grad_features, float* grid_inputs,
+                                         float* grad_w1, float* grad_b1, float* grad_w2, float* grad_b2,
+                                         void* stream) {
+    int st = prop_check(n_points, level_dim, num_levels, dtype, warp);
+    if (st) return st;
+    if (n_points == 0) return N3DS_OK;
+    const PropArgs P = prop_args(coords, n_points, warp, box_size, embeddings, offsets, num_levels, S, H, gridtype,
+                                 align_corners, interp, w1, b1, w2, b2, density_bias);
+    int64_t nb;
+    st = prop_blocks(n_points, PF_BWD_THREADS, 3, &nb);  // 3 resident blocks per CU (43 KB of LDS each)
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+#define LAUNCH_PB(CC, FM, FX)                                                                                    \
+    if (dtype == 0)                                                                                              \
+        hipLaunchKernelGGL((prop_bwd_kernel<float, CC, FM, FX>), dim3((unsigned)nb), dim3(PF_BWD_THREADS), 0, s, \
+                           P, grad_density, features, grad_features, grid_inputs, grad_w1, grad_b1, grad_w2,     \
+                           grad_b2);                                                                             \
+    else                                                                                                         \
+        hipLaunchKernelGGL((prop_bwd_kernel<__half, CC, FM, FX>), dim3((unsigned)nb), dim3(PF_BWD_THREADS), 0,   \
+                           s, P, grad_density, features, grad_features, grid_inputs, grad_w1, grad_b1, grad_w2,  \
+                           grad_b2);
+    const bool small = num_levels * level_dim <= 16;
+    if (level_dim == 1 && num_levels == 10) {
+        LAUNCH_PB(1, 12, 10)
+    } else if (level_dim == 2 && num_levels == 10) {
+        LAUNCH_PB(2, 20, 20)
+    } else if (level_dim == 1) {
+        if (small) { LAUNCH_PB(1, 16, 0) } else { LAUNCH_PB(1, 32, 0) }
+    } else if (level_dim == 2) {
+        if (small) { LAUNCH_PB(2, 16, 0) } else { LAUNCH_PB(2, 32, 0) }
+    } else {
+        if (small) { LAUNCH_PB(4, 16, 0) } else { LAUNCH_PB(4, 32, 0) }
+    }
+#undef LAUNCH_PB
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;
 }

@@ -323,6 +323,121 @@ def texture_block(seed=0, n_pairs=2048 * 128 * 500, steps=10):
                                 'grad_alphas': r(g_al, og[1]), 'grad_texcoords': r(g_tc, og[2])}}
 
 
+def dropin_block(seed=0, n_rays=2048, steps=5, check_points=256):
+    """The drop-in operators as the reference's StrokeField.predict_density calls them
+    (models.py:660-711): one stroke_fn per stroke type of the 500-stroke palette over the
+    points of 2048 rays x 128 samples, then compose_strokes('over') over all 500 strokes,
+    forward and adjoint (exact), each kernel timed with HIP events on its stream; GB/s of the
+    (point, stroke) tensors against 8 TB/s and errors against the float64 oracle on the first
+    check_points points."""
+    import torch
+    from neural3dstrokes_amd import scenes
+    from neural3dstrokes_amd.strokelib.strokes import _compositing_fn, _stroke_fn, set_adjoint
+    from oracle import compose as ocompose
+    from oracle import strokes as ostrokes
+    dev = torch.device('cuda:0')
+    sc = scenes.make_workload('palette500_65536x128', seed=seed, n_rays=n_rays)
+    R, n1 = sc['tdist'].shape
+    n = n1 - 1
+    o, d = sc['origins'].to(dev), sc['directions'].to(dev)
+    td = sc['tdist'].to(dev)
+    t = (td[:, 1:] + td[:, :-1]) / 2
+    x = (o[:, None, :] + d[:, None, :] * t[..., None]).contiguous()
+    radius = torch.full((R, n), 0.01, device=dev)
+    viewdir = sc['viewdirs'].to(dev)
+    ids = sc['stroke_ids'].tolist()
+    groups = {}
+    for k, sid in enumerate(ids):
+        groups.setdefault(sid, []).append(k)
+    sp_all, cp_all = sc['shape_params'].to(dev), sc['color_params'].to(dev)
+    dp = sc['density_params'].to(dev)
+    gd = torch.Generator(device=dev).manual_seed(seed)
+    set_adjoint('exact')
+    K = len(ids)
+    N = R * n
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    times = {'stroke_fwd': 0.0, 'stroke_bwd': 0.0, 'compose_fwd': 0.0, 'compose_bwd': 0.0}
+    per_type = {}
+    errs = {}
+    for rep in range(steps + 1):
+        alphas = torch.empty(R, n, K, device=dev)
+        colors = torch.empty(R, n, K, 3, device=dev)
+        lv = []
+        for sid, cols in groups.items():
+            P = scenes.n_shape_params(sid)
+            sp = sp_all[cols][:, :P].contiguous().requires_grad_(True)
+            cp = cp_all[cols].contiguous().requires_grad_(True)
+            e = [ev() for _ in range(4)]
+            e[0].record()
+            a, c, _, _ = _stroke_fn.apply(x, radius, viewdir, sp, cp, sid, 0, 1.0)
+            e[1].record()
+            ga = torch.randn(a.shape, device=dev, generator=gd)
+            gc = torch.randn(c.shape, device=dev, generator=gd)
+            e[2].record()
+            torch.autograd.backward([a, c], [ga, gc])
+            e[3].record()
+            lv.append((sid, cols, a, c, e, ga, gc, sp, cp))
+        for sid, cols, a, c, e, ga, gc, sp, cp in lv:
+            idx = torch.tensor(cols, device=dev)
+            alphas[..., idx] = a.detach()
+            colors[..., idx, :] = c.detach()
+        al = alphas.requires_grad_(True)
+        co = colors.requires_grad_(True)
+        dpl = dp.clone().requires_grad_(True)
+        ec = [ev() for _ in range(4)]
+        ec[0].record()
+        dens, col = _compositing_fn.apply(al, co, dpl)
+        ec[1].record()
+        gdn = torch.randn(dens.shape, device=dev, generator=gd)
+        gcl = torch.randn(col.shape, device=dev, generator=gd)
+        ec[2].record()
+        torch.autograd.backward([dens, col], [gdn, gcl])
+        ec[3].record()
+        torch.cuda.synchronize()
+        if rep == 0:  # warmup; errors against the float64 oracle on the first points
+            xs = x.reshape(-1, 3)[:check_points].double()
+            rs = radius.reshape(-1)[:check_points].double()
+            vs = viewdir.repeat_interleave(n, 0)[:check_points].double()
+            es = {'alpha': 0.0, 'color': 0.0}
+            for sid, cols, a, c, e, ga, gc, sp, cp in lv:
+                fw = ostrokes.stroke_forward(xs, rs, vs, sp.detach().double(), cp.detach().double(), sid, 0, 1.0,
+                                             False, False)
+                ao = a.detach().reshape(-1, len(cols))[:check_points]
+                co_ = c.detach().reshape(-1, len(cols), 3)[:check_points]
+                es['alpha'] = max(es['alpha'], rel(ao, fw['alpha']))
+                es['color'] = max(es['color'], rel(co_, fw['color']))
+            sub = slice(0, check_points)
+            ad = alphas.detach().reshape(-1, K)[sub].double()
+            cd = colors.detach().reshape(-1, K, 3)[sub].double()
+            dref, cref, _, _ = ocompose.compose_forward(ad, cd, dp.double())
+            es['compose_density'] = rel(dens.detach().reshape(-1)[sub], dref)
+            es['compose_color'] = rel(col.detach().reshape(-1, 3)[sub], cref)
+            errs = es
+            continue
+        for sid, cols, a, c, e, ga, gc, sp, cp in lv:
+            f_ms, b_ms = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+            times['stroke_fwd'] += f_ms / steps
+            times['stroke_bwd'] += b_ms / steps
+            pt = per_type.setdefault(str(sid), {'strokes': len(cols), 'fwd_ms': 0.0, 'bwd_ms': 0.0})
+            pt['fwd_ms'] += f_ms / steps
+            pt['bwd_ms'] += b_ms / steps
+        times['compose_fwd'] += ec[0].elapsed_time(ec[1]) / steps
+        times['compose_bwd'] += ec[2].elapsed_time(ec[3]) / steps
+    # (point, stroke) tensors: forward writes alpha + colour (16 B), the adjoint reads the
+    # upstream alpha / colour gradients and the saved alpha (20 B); compositing reads alpha +
+    # colour (16 B) forward and reads them again and writes their gradients (32 B) backward
+    b = {'stroke_fwd': N * K * 16, 'stroke_bwd': N * K * 20, 'compose_fwd': N * K * 16, 'compose_bwd': N * K * 32}
+    out = {'workload': f'{N} points (2048 rays x 128 samples) x 500 strokes of palette500_65536x128, one stroke_fn '
+                       f'per stroke type ({len(groups)} types), compose_strokes over all; exact adjoint',
+           'per_type_ms': per_type, 'rel_err_vs_fp64': errs}
+    for k, v in times.items():
+        out[k + '_ms'] = v
+        out[k + '_bytes'] = b[k]
+        out[k + '_GBps'] = b[k] / (v * 1e-3) / 1e9
+        out[k + '_frac_hbm'] = b[k] / (v * 1e-3) / 8e12
+    return out
+
+
 def stepfun_block(seed=0, n_rays=65536, n_bins=64, n_samples=128, steps=20):
     """Proposal-level resampling (stepfun.sample_intervals, single jitter) of 65536 rays from
     64 weighted bins to 128 intervals: kernel time, GB/s against 8 TB/s, the reference's torch
@@ -705,6 +820,7 @@ def main():
     ap.add_argument('--grid', type=int, default=1, help='hash-grid encoder block (0: skip)')
     ap.add_argument('--texture', type=int, default=1, help='image-texture block (0: skip)')
     ap.add_argument('--stepfun', type=int, default=1, help='proposal interval-resampling block (0: skip)')
+    ap.add_argument('--dropin', type=int, default=1, help='drop-in stroke / compositing operator block (0: skip)')
     ap.add_argument('--memory', type=int, default=1, help='1000-stroke 800x800x192 kept-vs-recompute block (0: skip)')
     ap.add_argument('--prop', type=int, default=1, help='fused proposal density field block (0: skip)')
     ap.add_argument('--multiscene', type=int, default=1, help='32-scene graph-captured training step block (0: skip)')
@@ -871,6 +987,8 @@ def main():
         out['image_texture'] = texture_block()
     if args.stepfun:
         out['interval_resampling'] = stepfun_block()
+    if args.dropin:
+        out['dropin_operators'] = dropin_block()
     if args.memory:
         out['memory_bounded'] = memory_block()
     if args.prop:

@@ -82,7 +82,9 @@ int n3ds_compose_forward(float* density_out, float* color_out, const float* alph
  * compositing.cu:61-157, host :206-263), called from strokes.py:348-373.
  * grad_alphas [N,S] and grad_colors [N,S,D] are written; grad_density_params
  * [S] is accumulated.  grad_density_out / grad_color_out may be NULL (zeros).
- * n_strokes <= 8192. */
+ * n_strokes <= 2^20 (beyond 8192 strokes the per-point transmittance is
+ * checkpointed every few chunks and recomputed, and grad_density_params is
+ * accumulated with global atomics). */
 int n3ds_compose_backward(float* grad_alphas, float* grad_colors, float* grad_density_params,
                           const float* grad_density_out, const float* grad_color_out, const float* alphas,
                           const float* colors, const float* density_params, int64_t n_points,

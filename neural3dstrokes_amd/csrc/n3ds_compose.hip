@@ -18,7 +18,8 @@
 namespace n3ds {
 
 constexpr int CWAVES = 4;
-constexpr int MAXCH = 128;  // chunks of 64 strokes per point (S <= 8192)
+constexpr int MAXCH = 128;          // transmittance checkpoints kept per point in the adjoint
+constexpr int LDS_GDP_MAX = 8192;   // strokes whose density gradients a block sums in LDS
 
 __device__ __forceinline__ float scan_prod_excl(float f, int lane) {
     float v = f;
@@ -73,13 +74,19 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
     float* __restrict__ g_alpha, float* __restrict__ g_colors, float* __restrict__ g_dp,
     const float* __restrict__ g_dens, const float* __restrict__ g_color, const float* __restrict__ alphas,
     const float* __restrict__ colors, const float* __restrict__ dp, int64_t N, int64_t S, int64_t ppb, int exact) {
-    extern __shared__ float lds_gdp[];  // [S]
+    extern __shared__ float lds_gdp[];  // [S] if S <= LDS_GDP_MAX, else unused (global atomics)
+    // transmittance entering chunk g * G (pass 1) and entering each chunk of the group being
+    // walked back to front (pass 2); G = 1 up to MAXCH chunks, so pass 2 recomputes nothing
     __shared__ float t_in[CWAVES][MAXCH];
+    __shared__ float t_grp[CWAVES][MAXCH];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int64_t i = threadIdx.x; i < S; i += 64 * CWAVES) lds_gdp[i] = 0.f;
+    const bool lds = S <= LDS_GDP_MAX;
+    if (lds)
+        for (int64_t i = threadIdx.x; i < S; i += 64 * CWAVES) lds_gdp[i] = 0.f;
     __syncthreads();
     const int64_t n0 = (int64_t)blockIdx.x * ppb, n1 = min(N, n0 + ppb);
     const int nch = (int)((S + 63) / 64);
+    const int G = (nch + MAXCH - 1) / MAXCH;
     for (int64_t n = n0 + wave; n < n1; n += CWAVES) {
         const float* arow = alphas + n * S;
         const float* crow = colors + n * S * D;
@@ -97,7 +104,7 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
             Dn += wave_sum(a != 0.f ? (v ? dp[k] : 0.f) * w : 0.f);
 #pragma unroll
             for (int c = 0; c < D; c++) C[c] += wave_sum(a != 0.f ? crow[(v ? k : 0) * D + c] * w : 0.f);
-            if (lane == 0) t_in[wave][ch] = T;
+            if (lane == 0 && ch % G == 0) t_in[wave][ch / G] = T;
             T = __shfl(Tk * f, 63, 64);
         }
         const float Tn = T;
@@ -115,6 +122,18 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
 #pragma unroll
         for (int c = 0; c < D; c++) Uc[c] = 0.f;
         for (int ch = nch - 1; ch >= 0; ch--) {
+            if (G > 1 && (ch % G == G - 1 || ch == nch - 1)) {
+                // entering a group from behind: its chunks' incoming transmittance, recomputed
+                // front to back from its checkpoint exactly as pass 1 formed it
+                const int g0 = (ch / G) * G;
+                float Tg = t_in[wave][ch / G];
+                for (int c2 = g0; c2 <= ch; c2++) {
+                    const int64_t k2 = S - 1 - ((int64_t)c2 * 64 + lane);
+                    const float f2 = 1.f - (k2 >= 0 ? arow[k2] : 0.f);
+                    if (lane == 0) t_grp[wave][c2 - g0] = Tg;
+                    Tg = __shfl(Tg * scan_prod_excl(f2, lane) * f2, 63, 64);
+                }
+            }
             int64_t k = S - 1 - ((int64_t)ch * 64 + lane);
             bool v = k >= 0;
             float a = v ? arow[k] : 0.f;
@@ -123,7 +142,8 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
             float ck[D];
 #pragma unroll
             for (int c = 0; c < D; c++) ck[c] = v ? crow[k * D + c] : 0.f;
-            float Tk = t_in[wave][ch] * scan_prod_excl(f, lane);
+            const float Tin = G == 1 ? t_in[wave][ch] : t_grp[wave][ch % G];
+            float Tk = Tin * scan_prod_excl(f, lane);
             // inclusive suffix composition (m, bd, bc) of lanes lane..63
             float m = f, bd = dk * a, bc[D];
 #pragma unroll
@@ -176,7 +196,7 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
                 g_alpha[n * S + k] = ga;
 #pragma unroll
                 for (int c = 0; c < D; c++) g_colors[(n * S + k) * D + c] = a > 0.f ? gC[c] * sc * w : 0.f;
-                if (a > 0.f) atomicAdd(&lds_gdp[k], gD * w);
+                if (a > 0.f) atomicAdd(lds ? &lds_gdp[k] : &g_dp[k], gD * w);
             }
             // carry: compose this whole chunk in front of the stroke composite behind it
             float m0 = __shfl(m, 0, 64), bd0 = __shfl(bd, 0, 64);
@@ -186,6 +206,7 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
             Rb = m0 * Rb;
         }
     }
+    if (!lds) return;
     __syncthreads();
     for (int64_t i = threadIdx.x; i < S; i += 64 * CWAVES) {
         float v = lds_gdp[i];
@@ -228,11 +249,12 @@ N3DS_EXPORT int n3ds_compose_backward(float* grad_alphas, float* grad_colors, fl
                                       void* stream) {
     N3DS_REQUIRE(color_dim == 1 || color_dim == 3, "color_dim must be 1 or 3 (got %lld)", (long long)color_dim);
     N3DS_REQUIRE(n_points >= 0 && n_strokes >= 0, "negative size");
-    N3DS_REQUIRE(n_strokes <= 64 * MAXCH, "n_strokes (%lld) exceeds %d", (long long)n_strokes, 64 * MAXCH);
+    N3DS_REQUIRE(n_strokes <= (int64_t)64 * MAXCH * MAXCH, "n_strokes (%lld) exceeds %d", (long long)n_strokes,
+                 64 * MAXCH * MAXCH);
     if (n_points == 0 || n_strokes == 0) return N3DS_OK;
     int64_t ppb = compose_ppb(n_points);
     dim3 grid((unsigned)((n_points + ppb - 1) / ppb));
-    size_t shm = (size_t)n_strokes * sizeof(float);
+    size_t shm = n_strokes <= LDS_GDP_MAX ? (size_t)n_strokes * sizeof(float) : 0;
     if (color_dim == 3)
         hipLaunchKernelGGL(compose_bwd_kernel<3>, grid, dim3(64 * CWAVES), shm, (hipStream_t)stream, grad_alphas,
                            grad_colors, grad_density_params, grad_density_out, grad_color_out, alphas, colors,

@@ -7,9 +7,10 @@
 // rotation are loaded once per lane and kept in registers; the point (x,
 // radius, view direction) is wave-uniform and broadcast.  Outputs
 // alpha[N,S] / color[N,S,3] are written lane-contiguous.  The adjoint keeps the
-// stroke's parameter gradients in registers across the whole point range and
-// issues one atomic per parameter per lane at the end; dL/dx is summed across
-// the wavefront before a single atomic per point.
+// stroke's parameter gradients in registers across the whole point range, adds
+// the block's wavefronts together in LDS and issues one atomic per parameter per
+// stroke per block (blocks are few and long: about 1024 over all stroke tiles);
+// dL/dx is summed across the wavefront before a single atomic per point.
 #include "n3ds_capi.h"
 #include "n3ds_stroke_common.h"
 
@@ -153,19 +154,44 @@ __global__ __launch_bounds__(BLOCK) void stroke_bwd_kernel(
             }
         }
     }
-    if (!kv) return;
+    // the block's wavefronts hold partial sums for the same 64 strokes: add them in LDS,
+    // then the first wavefront issues one global atomic per parameter per stroke
+    constexpr int NR = (NB > 0 ? NB : 1) + 10 + NQ;
+    __shared__ float red[NR * 64];
+    for (int i = threadIdx.x; i < NR * 64; i += BLOCK) red[i] = 0.f;
+    __syncthreads();
+    {
+        float v[NR];
+#pragma unroll
+        for (int i = 0; i < (NB > 0 ? NB : 1); i++) v[i] = NB > 0 ? gb[i] : 0.f;
+        constexpr int o = NB > 0 ? NB : 1;
+        v[o] = gs1;
+        v[o + 1] = gs3.x; v[o + 2] = gs3.y; v[o + 3] = gs3.z;
+        v[o + 4] = gang.x; v[o + 5] = gang.y; v[o + 6] = gang.z;
+        v[o + 7] = gtr.x; v[o + 8] = gtr.y; v[o + 9] = gtr.z;
+#pragma unroll
+        for (int i = 0; i < NQ; i++) v[o + 10 + i] = gc[i];
+#pragma unroll
+        for (int i = 0; i < NR; i++) atomicAdd(&red[i * 64 + lane], v[i]);
+    }
+    __syncthreads();
+    if (wave != 0 || !kv) return;
+    const float* r = red + lane;
+    constexpr int o = NB > 0 ? NB : 1;
     float* grow = gshape + k * P;
 #pragma unroll
-    for (int i = 0; i < NB; i++) atomicAdd(grow + i, gb[i]);
+    for (int i = 0; i < NB; i++) atomicAdd(grow + i, r[i * 64]);
     if (F.off_scale >= 0) {
-        if (flags & F_SS) atomicAdd(grow + F.off_scale, gs1);
-        else { atomicAdd(grow + F.off_scale, gs3.x); atomicAdd(grow + F.off_scale + 1, gs3.y); atomicAdd(grow + F.off_scale + 2, gs3.z); }
+        if (flags & F_SS) atomicAdd(grow + F.off_scale, r[o * 64]);
+        else for (int i = 0; i < 3; i++) atomicAdd(grow + F.off_scale + i, r[(o + 1 + i) * 64]);
     }
-    if (F.off_angle >= 0) { atomicAdd(grow + F.off_angle, gang.x); atomicAdd(grow + F.off_angle + 1, gang.y); atomicAdd(grow + F.off_angle + 2, gang.z); }
-    if (F.off_trans >= 0) { atomicAdd(grow + F.off_trans, gtr.x); atomicAdd(grow + F.off_trans + 1, gtr.y); atomicAdd(grow + F.off_trans + 2, gtr.z); }
+    if (F.off_angle >= 0)
+        for (int i = 0; i < 3; i++) atomicAdd(grow + F.off_angle + i, r[(o + 4 + i) * 64]);
+    if (F.off_trans >= 0)
+        for (int i = 0; i < 3; i++) atomicAdd(grow + F.off_trans + i, r[(o + 7 + i) * 64]);
     float* gcrow = gcolor + k * Q;
 #pragma unroll
-    for (int i = 0; i < NQ; i++) atomicAdd(gcrow + i, gc[i]);
+    for (int i = 0; i < NQ; i++) atomicAdd(gcrow + i, r[(o + 10 + i) * 64]);
 }
 
 typedef void (*FwdFn)(float*, float*, float*, float*, const float*, const float*, const float*, const float*,
@@ -187,11 +213,12 @@ static const FwdFn kFwd[NUM_BASE][4] = {N3DS_ROW(0), N3DS_ROW(1), N3DS_ROW(2), N
 static const BwdFn kBwd[NUM_BASE][4] = {N3DS_ROWB(0), N3DS_ROWB(1), N3DS_ROWB(2), N3DS_ROWB(3), N3DS_ROWB(4), N3DS_ROWB(5),
                                         N3DS_ROWB(6), N3DS_ROWB(7), N3DS_ROWB(8), N3DS_ROWB(9), N3DS_ROWB(10), N3DS_ROWB(11)};
 
-static int64_t points_per_block(int64_t N, int64_t S) {
+// points per block for about target_blocks blocks over the stroke tiles (the adjoint
+// uses fewer, longer blocks: each block ends with an atomic per stroke parameter)
+static int64_t points_per_block(int64_t N, int64_t S, int64_t target_blocks = 4096) {
     int64_t tiles = (S + 63) / 64;
-    int64_t target_blocks = 4096;
     int64_t ppb = (N * tiles + target_blocks - 1) / target_blocks;
-    ppb = ppb < 64 ? 64 : (ppb > 4096 ? 4096 : ppb);
+    ppb = ppb < 64 ? 64 : (ppb > 16384 ? 16384 : ppb);
     return ((ppb + WAVES - 1) / WAVES) * WAVES;
 }
 
@@ -244,7 +271,7 @@ N3DS_EXPORT int n3ds_stroke_backward(float* grad_shape_params, float* grad_color
     int st = check_common(n_points, n_viewdirs, n_strokes, n_shape_params, n_color_params, sdf_id, color_id);
     if (st) return st;
     if (n_points == 0 || n_strokes == 0) return N3DS_OK;
-    int64_t ppb = points_per_block(n_points, n_strokes);
+    int64_t ppb = points_per_block(n_points, n_strokes, 1024);
     dim3 grid((unsigned)((n_points + ppb - 1) / ppb), (unsigned)((n_strokes + 63) / 64));
     hipLaunchKernelGGL(kBwd[sdf_id >> 4][color_id], grid, dim3(BLOCK), 0, (hipStream_t)stream, grad_shape_params,
                        grad_color_params, grad_x, grad_alpha, grad_color, grad_sdf, x, radius, viewdir, alpha,

@@ -56,6 +56,44 @@ def test_compose_backward(S, adjoint, D, dev):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('S', [8200, 20000])
+@pytest.mark.parametrize('adjoint', ['exact', 'reference'])
+def test_compose_backward_many_strokes(S, adjoint, dev):
+    """Beyond 8192 strokes per point (no limit in the reference): checkpointed transmittance and
+    density-parameter gradients accumulated in global memory.  Small alphas keep the strokes at the
+    back visible (total optical depth about 1)."""
+    n = 24
+    g = torch.Generator().manual_seed(S)
+    a = torch.rand(n, S, generator=g) * (4.0 / S)
+    a = torch.where(torch.rand(n, S, generator=g) < 0.5, torch.zeros_like(a), a)
+    c = torch.rand(n, S, 3, generator=g)
+    d = torch.exp(torch.rand(S, generator=g) * 4.6)
+    gd = rand_like_seeded((n,), S)
+    gc = rand_like_seeded((n, 3), S + 1)
+    ad, cd, dd = (t.to(dev).requires_grad_(True) for t in (a, c, d))
+    dstrokes.set_adjoint(adjoint)
+    try:
+        dens, col = dstrokes.compose_strokes(ad, cd, dd, 'over')
+        ((dens * gd.to(dev)).sum() + (col * gc.to(dev)).sum()).backward()
+    finally:
+        dstrokes.set_adjoint('exact')
+    rd, rcol, _, _ = ocompose.compose_forward(a.double(), c.double(), d.double())
+    # float32 running sums over S / 64 chunks of about 32 nonzero terms: relative error ~ S / 64 ulp
+    assert rel_err(dens, rd) < 5e-5 and rel_err(col, rcol) < 5e-5
+    if adjoint == 'exact':  # autograd of the closed form (cumulative product), the same derivative
+        leaves = [t.double().requires_grad_(True) for t in (a, c, d)]
+        cd_, cc_ = ocompose.compose_closed_form(*leaves)
+        torch.autograd.backward([cd_, cc_], [gd.double(), gc.double()])
+        ra, rc, rdp = (t.grad for t in leaves)
+    else:
+        ra, rc, rdp = ocompose.compose_backward_reference(gd.double(), gc.double(), a.double(), c.double(),
+                                                          d.double())
+    assert rel_err(ad.grad, ra) < 1e-4
+    assert rel_err(cd.grad, rc) < 1e-4
+    assert rel_err(dd.grad, rdp) < 1e-4
+
+
+@pytest.mark.gpu
 def test_compose_missing_grad_is_zero(dev):
     """An absent upstream grad (empty tensor in the reference) is treated as zeros."""
     a, c, d = _inputs(64, 7, 3, 5)

@@ -223,6 +223,20 @@ int n3ds_render_forward(const float* origins, const float* directions, const flo
                         float* pair_buf, int cap_hits, int cap_pairs, void* workspace, int64_t workspace_bytes,
                         void* stream);
 
+/* Eval-time extras of the ray march: replaces the compute_extras branch of
+ * volumetric_rendering (source/utils/render.py:88-116, called from
+ * models.py:224-233 when render_image asks for extras) for the fused
+ * renderer.  From the per-sample composite density in state_out of
+ * n3ds_render_forward (row 0 of [5, R*n]), tdist [R, n+1], directions [R,3]
+ * and the far distance t_far [R] (batch['far']): extras_out [R, 4] =
+ * distance_mean (exp of the weighted mean log depth, clipped to the ray's
+ * fenceposts), and the 5th, 50th and 95th weighted percentiles of the
+ * fenceposts extended by t_far with the background weight
+ * (stepfun.weighted_percentile, stepfun.py:351-361).  opaque_background as
+ * passed to the forward. */
+int n3ds_render_extras(const float* state, const float* tdist, const float* directions, const float* t_far,
+                       int64_t n_rays, int n_samples, int opaque_background, float* extras_out, void* stream);
+
 /* Launch-shape parameters of the render kernels (no counterpart in the
  * reference), set by the autotuner for the current process: 0 = pending
  * (sample, stroke) pairs that trigger an evaluation window (16..256, default

@@ -1468,6 +1468,109 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
 }
 
 // ---------------------------------------------------------------------------
+// eval-time extras of the ray march
+// ---------------------------------------------------------------------------
+// volumetric_rendering(compute_extras=True) (source/utils/render.py:88-116) from
+// the forward's per-sample composite density (state row 0): the weights as the
+// forward's ray march forms them (compute_alpha_weights, render.py:29-48), the
+// log-distance mean, and the 5 / 50 / 95 weighted percentiles of the fenceposts
+// extended by the far distance with the background weight
+// (stepfun.weighted_percentile over integrate_weights, located by sorted_interp,
+// stepfun.py:5-24,149-170,351-361).  For a percentile x the fenceposts split by
+// cw_j <= x; the interval is (largest cw / t of the first set, smallest of the
+// second), defaulting to (cw, t) = (0, t_0) and (1, t_far).  One wavefront per
+// ray; out [R, 4] = distance mean, 5th percentile, median, 95th percentile.
+constexpr int XWAVES = 4;
+
+__global__ __launch_bounds__(64 * XWAVES) void render_extras_kernel(const float* __restrict__ state,
+                                                                    const float* __restrict__ tdist,
+                                                                    const float* __restrict__ dirs,
+                                                                    const float* __restrict__ t_far, int64_t R,
+                                                                    int n, int opaque, float* __restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ray = (int64_t)blockIdx.x * XWAVES + wave;
+    if (ray >= R) return;
+    const float* td = tdist + ray * (n + 1);
+    const float* dens = state + ray * n;
+    const V3 d = ld3(dirs + 3 * ray);
+    const float dn = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    const float far = t_far[ray];
+    const float ps[3] = {0.05f, 0.5f, 0.95f};
+    float x0[3], x1[3], f0[3], f1[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        x0[q] = 0.f;
+        f0[q] = td[0];
+        x1[q] = 1.f;
+        f1[q] = far;
+    }
+    float carry = 0.f, wcarry = 0.f, acc = 0.f, lsum = 0.f;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int i = c0 + lane;
+        const bool valid = i < n;
+        const float ta = valid ? td[i] : 0.f, tb = valid ? td[i + 1] : 0.f;
+        const float sd = valid ? dens[i] * ((tb - ta) * dn) : 0.f;
+        float v = sd;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const float u = __shfl_up(v, off, 64);
+            if (lane >= off) v += u;
+        }
+        const float trans = expf(-(carry + v - sd));
+        const float al = (opaque && i == n - 1) ? 1.f : 1.f - expf(-sd);
+        const float w = valid ? al * trans : 0.f;
+        carry += __shfl(v, 63, 64);
+        acc += w;
+        if (valid) lsum += w * logf(0.5f * (ta + tb));
+        // fencepost i + 1: the integrated weight of samples 0..i (clamped to 1) at depth td[i + 1]
+        float cw = w;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const float u = __shfl_up(cw, off, 64);
+            if (lane >= off) cw += u;
+        }
+        const float cwi = fminf(wcarry + cw, 1.f);
+        wcarry += __shfl(cw, 63, 64);
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                if (ps[q] >= cwi) {
+                    x0[q] = fmaxf(x0[q], cwi);
+                    f0[q] = fmaxf(f0[q], tb);
+                } else {
+                    x1[q] = fminf(x1[q], cwi);
+                    f1[q] = fminf(f1[q], tb);
+                }
+            }
+        }
+    }
+    acc = wave_sum(acc);
+    lsum = wave_sum(lsum);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            x0[q] = fmaxf(x0[q], __shfl_xor(x0[q], off, 64));
+            f0[q] = fmaxf(f0[q], __shfl_xor(f0[q], off, 64));
+            x1[q] = fminf(x1[q], __shfl_xor(x1[q], off, 64));
+            f1[q] = fminf(f1[q], __shfl_xor(f1[q], off, 64));
+        }
+    }
+    if (lane == 0) {
+        float dm = expf(lsum / fmaxf(acc, 1.1920929e-07f));
+        if (isnan(dm)) dm = INFINITY;
+        out[4 * ray] = fminf(fmaxf(dm, td[0]), td[n]);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            float o = (ps[q] - x0[q]) / (x1[q] - x0[q]);
+            if (isnan(o)) o = 0.f;
+            o = fminf(fmaxf(o, 0.f), 1.f);
+            out[4 * ray + 1 + q] = f0[q] + o * (f1[q] - f0[q]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // backward
 // ---------------------------------------------------------------------------
 // canonical gradient slot -> column in the [Pmax | Q | 1] accumulator row (-1: none)
@@ -2250,6 +2353,19 @@ static int launch_cull(RenderArgs& A, int ns, void* workspace, int64_t workspace
     N3DS_LAUNCH_CHECK();
     A.ccounts = counts;
     A.cslots = slots;
+    return N3DS_OK;
+}
+
+N3DS_EXPORT int n3ds_render_extras(const float* state, const float* tdist, const float* directions,
+                                   const float* t_far, int64_t n_rays, int n_samples, int opaque_background,
+                                   float* extras_out, void* stream) {
+    N3DS_REQUIRE(n_rays >= 0 && n_samples >= 1, "bad ray sizes");
+    N3DS_REQUIRE(n_rays == 0 || (state && tdist && directions && t_far && extras_out), "a required pointer is NULL");
+    if (n_rays == 0) return N3DS_OK;
+    hipLaunchKernelGGL(render_extras_kernel, dim3((unsigned)((n_rays + XWAVES - 1) / XWAVES)), dim3(64 * XWAVES), 0,
+                       (hipStream_t)stream, state, tdist, directions, t_far, n_rays, n_samples, opaque_background,
+                       extras_out);
+    N3DS_LAUNCH_CHECK();
     return N3DS_OK;
 }
 

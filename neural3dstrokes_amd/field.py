@@ -30,7 +30,7 @@ import torch.nn as nn
 
 from . import render as _render
 from .checkpoint import load_state, stroke_sdf_id
-from .propfield import PropDensityField
+from .propfield import PropDensityField, prop_field
 from .strokelib.strokes import _color_dict, get_stroke
 
 
@@ -62,7 +62,6 @@ class ErrorField(PropDensityField):
         """Error density at world-space points [..., 3] (``models.py:512-523``)."""
         prefix = coords.shape[:-1]
         lin0, lin2 = self.density_layer[0], self.density_layer[2]
-        from .propfield import prop_field
         density, _ = prop_field(coords.reshape(-1, 3), self.encoder, lin0.weight, lin0.bias, lin2.weight, lin2.bias,
                                 self.density_bias, self.warp_fn, self.bbox_size, no_warp)
         return density.view(prefix)
@@ -214,15 +213,31 @@ class StrokeField(nn.Module):
     def render(self, origins, directions, viewdirs, radii, tdist, bg_color=(1.0, 1.0, 1.0),
                opaque_background=False, **kw):
         """rgb [R,3], acc [R], depth [R] of rays through the field (radii [R] or [R,1], tdist [R, n+1])."""
+        return _render.render_strokes(*self._render_args(origins, directions, viewdirs, radii, tdist),
+                                      bg_color=bg_color, opaque_background=opaque_background, **self._render_kw(),
+                                      **kw)
+
+    def render_eval(self, origins, directions, viewdirs, radii, tdist, t_far, compute_extras=True,
+                    bg_color=(1.0, 1.0, 1.0), opaque_background=False, **kw):
+        """The reference's rendering dict of an eval pass (no gradients): rgb, depth, acc and the
+        compute_extras distances (``utils/render.py:88-116``; t_far = batch['far'])."""
+        return _render.render_rays(*self._render_args(origins, directions, viewdirs, radii, tdist), t_far=t_far,
+                                   compute_extras=compute_extras, bg_color=bg_color,
+                                   opaque_background=opaque_background, **self._render_kw(), **kw)
+
+    def _render_kw(self):
+        return dict(color_id=self.color_id, sdf_delta=self.current_sdf_delta(),
+                    use_laplace_transform=self.use_laplace_transform, inv_scale_radius=self.inv_scale_radius,
+                    coord_div=self.bbox_size / 2, cone=not self.no_adaptive_delta)
+
+    def _render_args(self, origins, directions, viewdirs, radii, tdist):
         if self.warp_fn is not None:
             raise NotImplementedError("the fused renderer takes the linear warp (warp_fn=None) only")
         fixed, step = self.stroke_range()
         radii = radii.reshape(-1).float()
-        cone = True
-        if self.no_adaptive_delta:  # models.py:728-729: every sample gets the mean cone radius
+        if self.no_adaptive_delta:  # models.py:728-729: every sample gets the mean cone radius (cone off)
             t = (tdist[:, 1:] + tdist[:, :-1]) / 2
             radii = torch.full_like(radii, float((radii[:, None] * t).mean()))
-            cone = False
         sp, cp = self.shape_params[:step], self.color_params[:step]
         dp = self.density_params[:step] * self.density_scale
         if fixed > 0:  # strokes behind the optimised window take no gradient (models.py:686-697)
@@ -230,12 +245,7 @@ class StrokeField(nn.Module):
             cp = torch.cat([cp[:fixed].detach(), cp[fixed:]])
             dp = torch.cat([dp[:fixed].detach(), dp[fixed:]])
         ids = torch.full((step,), self.sdf_id, dtype=torch.int32)
-        return _render.render_strokes(origins, directions, viewdirs, radii, tdist, ids, sp, cp, dp,
-                                      color_id=self.color_id, sdf_delta=self.current_sdf_delta(),
-                                      use_laplace_transform=self.use_laplace_transform,
-                                      inv_scale_radius=self.inv_scale_radius, cone=cone,
-                                      coord_div=self.bbox_size / 2, bg_color=bg_color,
-                                      opaque_background=opaque_background, **kw)
+        return origins, directions, viewdirs, radii, tdist, ids, sp, cp, dp
 
 
 def load_model_state(path):

@@ -90,62 +90,73 @@ def _keep_buffers(store, R, n, S, color_id, dev):
             torch.empty(R * npf * cap_p, dtype=torch.float32, device=dev), cap_h, cap_p)
 
 
+def _launch_forward(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
+                    density_params, cfg, need_grad):
+    """The forward launch: (rgb, acc, depth, per-sample state [5, R*n], stroke table, per-ray
+    backgrounds or None, host background colour, kept buffers or None)."""
+    (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store, scenes,
+     opaque) = cfg
+    R, n1 = tdist.shape
+    S, Pmax = shape_params.shape
+    if scenes is not None:
+        scenes.check(R, n1 - 1, S, Pmax, color_params.shape[1], color_id)
+    for name, t in (('origins', origins), ('directions', directions), ('viewdirs', viewdirs), ('radii', radii),
+                    ('tdist', tdist), ('shape_params', shape_params), ('color_params', color_params),
+                    ('density_params', density_params)):
+        _lib.require_cuda(name, t)
+    _lib.require_cuda('stroke_ids', stroke_ids, dtype=torch.int32)
+    dev = origins.device
+    table, status = prepare_table(stroke_ids, shape_params)
+    _check_device_ids(stroke_ids, Pmax, status)
+    bg_rays = None
+    if torch.is_tensor(bg) and bg.dim() == 2:  # per-ray background colours [R, 3]
+        bg_rays = bg.to(device=origins.device, dtype=torch.float32).contiguous()
+        if bg_rays.shape != (R, 3):
+            raise RuntimeError(f'bg_color has shape {tuple(bg_rays.shape)}, per-ray colours need ({R}, 3)')
+        bgt = torch.ones(3)
+    else:
+        bgt = torch.as_tensor(bg, dtype=torch.float32).cpu()
+    bgv = (_lib.ctypes.c_float * 3)(*[float(v) for v in bgt.reshape(-1).expand(3)])
+    rgb = torch.empty(R, 3, dtype=torch.float32, device=dev)
+    acc = torch.empty(R, dtype=torch.float32, device=dev)
+    depth = torch.empty(R, dtype=torch.float32, device=dev)
+    state = torch.empty(5, R * (n1 - 1), dtype=torch.float32, device=dev)
+    s_keep = S if scenes is None else scenes.s_max  # kept-hit layout follows the largest scene
+    keep = _keep_buffers(store, R, n1 - 1, s_keep, color_id, dev) if need_grad else None
+    hd, hb, pb, cap_h, cap_p = keep if keep is not None else (None, None, None, 0, 0)
+    if scenes is None:
+        fwd, batch_args = _lib.lib().n3ds_render_forward, ()
+    else:
+        fwd, batch_args = _lib.lib().n3ds_render_batch_forward, scenes.c_args()
+    # culling as its own pass (260 B per (ray, chunk of 64 strokes)) unless the caller
+    # asked for the low-memory mode
+    cws = None
+    if store is not False:
+        nbytes = int(_lib.lib().n3ds_render_forward_workspace_bytes(R, s_keep))
+        cws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+    _lib.check(fwd(
+        _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
+        _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax,
+        color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
+        _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), _lib.ptr(bg_rays), int(opaque), float(cull_eps),
+        _lib.ptr(rgb), _lib.ptr(acc),
+        _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.ptr(hd), _lib.ptr(hb), _lib.ptr(pb), cap_h,
+        cap_p, _lib.ptr(cws), 0 if cws is None else int(cws.numel()), _lib.stream()))
+    del cws
+    return rgb, acc, depth, state, table, bg_rays, bgv, keep
+
+
 class _render_fn(Function):
 
     @staticmethod
     def forward(ctx, origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
                 density_params, cfg):
-        (color_id, sdf_delta, laplace, isr, cone, coord_div, bg, cull_eps, adjoint, counters, store, scenes,
-         opaque) = cfg
-        R, n1 = tdist.shape
-        S, Pmax = shape_params.shape
-        if scenes is not None:
-            scenes.check(R, n1 - 1, S, Pmax, color_params.shape[1], color_id)
-        for name, t in (('origins', origins), ('directions', directions), ('viewdirs', viewdirs), ('radii', radii),
-                        ('tdist', tdist), ('shape_params', shape_params), ('color_params', color_params),
-                        ('density_params', density_params)):
-            _lib.require_cuda(name, t)
-        _lib.require_cuda('stroke_ids', stroke_ids, dtype=torch.int32)
-        dev = origins.device
-        table, status = prepare_table(stroke_ids, shape_params)
-        _check_device_ids(stroke_ids, Pmax, status)
-        bg_rays = None
-        if torch.is_tensor(bg) and bg.dim() == 2:  # per-ray background colours [R, 3]
-            bg_rays = bg.to(device=origins.device, dtype=torch.float32).contiguous()
-            if bg_rays.shape != (R, 3):
-                raise RuntimeError(f'bg_color has shape {tuple(bg_rays.shape)}, per-ray colours need ({R}, 3)')
-            bgt = torch.ones(3)
-        else:
-            bgt = torch.as_tensor(bg, dtype=torch.float32).cpu()
-        bgv = (_lib.ctypes.c_float * 3)(*[float(v) for v in bgt.reshape(-1).expand(3)])
-        rgb = torch.empty(R, 3, dtype=torch.float32, device=dev)
-        acc = torch.empty(R, dtype=torch.float32, device=dev)
-        depth = torch.empty(R, dtype=torch.float32, device=dev)
-        state = torch.empty(5, R * (n1 - 1), dtype=torch.float32, device=dev)
         need_grad = any(ctx.needs_input_grad[6:9])
-        s_keep = S if scenes is None else scenes.s_max  # kept-hit layout follows the largest scene
-        keep = _keep_buffers(store, R, n1 - 1, s_keep, color_id, dev) if need_grad else None
+        (rgb, acc, depth, state, table, bg_rays, bgv, keep) = _launch_forward(
+            origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params, density_params, cfg,
+            need_grad)
         hd, hb, pb, cap_h, cap_p = keep if keep is not None else (None, None, None, 0, 0)
-        if scenes is None:
-            fwd, batch_args = _lib.lib().n3ds_render_forward, ()
-        else:
-            fwd, batch_args = _lib.lib().n3ds_render_batch_forward, scenes.c_args()
-        # culling as its own pass (260 B per (ray, chunk of 64 strokes)) unless the caller
-        # asked for the low-memory mode
-        cws = None
-        if store is not False:
-            nbytes = int(_lib.lib().n3ds_render_forward_workspace_bytes(R, s_keep))
-            cws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
-        _lib.check(fwd(
-            _lib.ptr(origins), _lib.ptr(directions), _lib.ptr(viewdirs), _lib.ptr(radii), _lib.ptr(tdist),
-            _lib.ptr(table), _lib.ptr(color_params), _lib.ptr(density_params), *batch_args, R, n1 - 1, S, Pmax,
-            color_params.shape[1], color_id, float(sdf_delta), int(laplace), int(isr), int(cone), float(coord_div),
-            _lib.ctypes.cast(bgv, _lib.ctypes.c_void_p), _lib.ptr(bg_rays), int(opaque), float(cull_eps),
-            _lib.ptr(rgb), _lib.ptr(acc),
-            _lib.ptr(depth), _lib.ptr(state), _lib.ptr(counters), _lib.ptr(hd), _lib.ptr(hb), _lib.ptr(pb), cap_h,
-            cap_p, _lib.ptr(cws), 0 if cws is None else int(cws.numel()), _lib.stream()))
-        del cws
-        empty = torch.empty(0, device=dev)
+        empty = torch.empty(0, device=origins.device)
         ctx.save_for_backward(origins, directions, viewdirs, radii, tdist, table, shape_params, color_params,
                               density_params, state, hd if hd is not None else empty,
                               hb if hb is not None else empty, pb if pb is not None else empty,
@@ -228,6 +239,17 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
     independent scenes packed one after another (each ray composites only its
     own scene's strokes); every stage then runs as one launch for all scenes.
     """
+    return _render_fn.apply(*_render_args(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params,
+                                          color_params, density_params, color_id, sdf_delta, use_laplace_transform,
+                                          inv_scale_radius, cone, coord_div, bg_color, cull_eps, adjoint, counters,
+                                          store, scenes, opaque_background))
+
+
+def _render_args(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
+                 density_params, color_id=0, sdf_delta=1.0, use_laplace_transform=False, inv_scale_radius=False,
+                 cone=False, coord_div=1.0, bg_color=(1.0, 1.0, 1.0), cull_eps=1e-7, adjoint=None,
+                 counters=None, store='auto', scenes=None, opaque_background=False):
+    """Checked, contiguous float32 arguments and the configuration tuple of the render op."""
     if color_params.shape[1] != COLOR_WIDTH[color_id]:
         raise RuntimeError(f'color_params has width {color_params.shape[1]}, color_id {color_id} needs '
                            f'{COLOR_WIDTH[color_id]}')
@@ -245,6 +267,38 @@ def render_strokes(origins, directions, viewdirs, radii, tdist, stroke_ids, shap
     cfg = (int(color_id), float(sdf_delta), bool(use_laplace_transform), bool(inv_scale_radius), bool(cone),
            float(coord_div), bg_color, float(cull_eps), adjoint or get_adjoint(), counters, store, scenes,
            bool(opaque_background))
-    f = lambda t: t.contiguous().float()
-    return _render_fn.apply(f(origins), f(directions), f(viewdirs), f(radii), f(tdist), stroke_ids.contiguous(),
-                            f(shape_params), f(color_params), f(density_params), cfg)
+    f = lambda t: t.contiguous().float()  # noqa: E731
+    args = (f(origins), f(directions), f(viewdirs), f(radii), f(tdist), stroke_ids.contiguous(), f(shape_params),
+            f(color_params), f(density_params), cfg)
+    return args
+
+
+def render_rays(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
+                density_params, t_far=None, compute_extras=False, **kw):
+    """Eval-time rendering (no gradients) with the reference's rendering dict
+    (``volumetric_rendering``, ``utils/render.py:51-116``): 'rgb', 'depth', 'acc' and, with
+    ``compute_extras``, 'distance_mean', 'distance_median', 'distance_percentile_5' and
+    'distance_percentile_95' (fenceposts extended by the far distance ``t_far``, [R] or a
+    number, ``batch['far']``) from the fused forward's per-sample state
+    (``n3ds_render_extras``).  Other arguments as ``render_strokes``."""
+    R = tdist.shape[0]
+    if compute_extras and t_far is None:
+        raise ValueError('compute_extras needs t_far (the far distance of each ray)')
+    args = _render_args(origins, directions, viewdirs, radii, tdist, stroke_ids, shape_params, color_params,
+                        density_params, **kw)
+    with torch.no_grad():
+        rgb, acc, depth, state, _, _, _, _ = _launch_forward(*args, need_grad=False)
+        out = {'rgb': rgb, 'depth': depth, 'acc': acc}
+        if compute_extras:
+            far = torch.as_tensor(t_far, dtype=torch.float32, device=rgb.device).reshape(-1)
+            far = far.expand(R).contiguous() if far.numel() == 1 else far.contiguous()
+            if far.numel() != R:
+                raise RuntimeError(f't_far has {far.numel()} values for {R} rays')
+            ex = torch.empty(R, 4, dtype=torch.float32, device=rgb.device)
+            _lib.check(_lib.lib().n3ds_render_extras(_lib.ptr(state), _lib.ptr(args[4]), _lib.ptr(args[1]),
+                                                     _lib.ptr(far), R, tdist.shape[1] - 1, int(args[9][12]),
+                                                     _lib.ptr(ex), _lib.stream()))
+            for i, k in enumerate(('distance_mean', 'distance_percentile_5', 'distance_median',
+                                   'distance_percentile_95')):
+                out[k] = ex[:, i]
+    return out

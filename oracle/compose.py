@@ -10,7 +10,10 @@ colour and ``D = d·α``; all-zero α gives ``D = 0`` and ``C = 0``; the sequent
 over-operator equals the closed form with a reversed exclusive ``cumprod``; the
 ray march equals ``1 - exp`` / ``cumsum`` closed forms and ``Σw ≤ 1``; the
 reference adjoint's density part equals the autograd derivative while its
-colour part differs (code-evident, only equal for one stroke).
+colour part differs (code-evident, only equal for one stroke).  The eval-time
+extras (``volumetric_extras``) are pinned by ``numpy.interp`` on strictly
+increasing integrated weights, closed forms for one opaque sample, an empty
+ray and uniform weights (``tests/test_oracle_extras.py``).
 """
 import torch
 
@@ -167,3 +170,48 @@ def volumetric_rendering(rgbs, weights, tdist, bg_rgbs):
     depth = torch.clip(torch.nan_to_num((weights * t_mids).sum(dim=-1) / acc.clamp_min(eps), torch.inf),
                        tdist[..., 0], tdist[..., -1])
     return rgb, acc, depth
+
+
+def sorted_interp(x, xp, fp):
+    """stepfun.py:5-24.  For each query x (sorted xp): the fenceposts with xp <= x and the rest;
+    (x0, f0) = the largest xp and fp of the first set, (x1, f1) = the smallest of the second
+    (the first / last entry when a set is empty), then f0 + clip((x - x0) / (x1 - x0)) (f1 - f0)
+    with 0 / 0 -> 0."""
+    below = xp[..., :, None] <= x[..., None, :]                     # [..., m, k]
+    first = lambda v: v[..., :1, None].expand(below.shape)          # noqa: E731
+    last = lambda v: v[..., -1:, None].expand(below.shape)          # noqa: E731
+    xv = xp[..., :, None].expand(below.shape)
+    fv = fp[..., :, None].expand(below.shape)
+    x0 = torch.where(below, xv, first(xp)).amax(-2)
+    f0 = torch.where(below, fv, first(fp)).amax(-2)
+    x1 = torch.where(below, last(xp), xv).amin(-2)
+    f1 = torch.where(below, last(fp), fv).amin(-2)
+    frac = torch.nan_to_num((x - x0) / (x1 - x0), 0.0).clamp(0.0, 1.0)
+    return f0 + frac * (f1 - f0)
+
+
+def weighted_percentile(t, w, ps):
+    """stepfun.py:351-361 over integrate_weights (stepfun.py:149-170): [..., len(ps)]."""
+    cw = torch.cumsum(w[..., :-1], dim=-1).clamp_max(1)
+    pad = cw.shape[:-1] + (1,)
+    cw0 = torch.cat([torch.zeros(pad, dtype=w.dtype, device=w.device), cw,
+                     torch.ones(pad, dtype=w.dtype, device=w.device)], -1)
+    x = (torch.tensor(ps, dtype=w.dtype, device=w.device) / 100).expand(cw0.shape[:-1] + (len(ps),))
+    return sorted_interp(x.contiguous(), cw0, t)
+
+
+def volumetric_extras(weights, tdist, t_far, eps=None):
+    """render.py:98-116 (compute_extras): distance_mean and the 5 / 50 / 95 percentiles of the
+    fenceposts extended by t_far [R] with the background weight.  eps: the clamp of acc (the
+    reference's is float32's epsilon)."""
+    eps = torch.finfo(torch.float32).eps if eps is None else eps
+    acc = weights.sum(-1)
+    t_mids = 0.5 * (tdist[..., :-1] + tdist[..., 1:])
+    mean_log = (weights * torch.log(t_mids)).sum(-1) / acc.clamp_min(eps)
+    dmean = torch.clip(torch.nan_to_num(torch.exp(mean_log), torch.inf), tdist[..., 0], tdist[..., -1])
+    bg_w = (1 - acc[..., None]).clamp_min(0.)
+    t_aug = torch.cat([tdist, t_far.reshape(-1, 1).to(tdist)], -1)
+    w_aug = torch.cat([weights, bg_w], -1)
+    pc = weighted_percentile(t_aug, w_aug, [5, 50, 95])
+    return {'distance_mean': dmean, 'distance_percentile_5': pc[..., 0], 'distance_median': pc[..., 1],
+            'distance_percentile_95': pc[..., 2]}

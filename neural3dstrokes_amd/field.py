@@ -118,6 +118,18 @@ class StrokeField(nn.Module):
         self.register_buffer('shape_params_grad', torch.zeros(self.max_num_strokes, self.d_shape))
         self.stroke_step_limit = None
         self.last_update_step = 0
+        self._host_step = None  # host copy of stroke_step (None: read it from the device once)
+        self._ids = None        # device stroke ids, built once per device
+
+    def _load_from_state_dict(self, *args, **kwargs):
+        self._host_step = None
+        super()._load_from_state_dict(*args, **kwargs)
+
+    def n_strokes(self):
+        """stroke_step as a host int, read from the device only after it changes outside step_update."""
+        if self._host_step is None:
+            self._host_step = int(self.stroke_step.item())
+        return self._host_step
 
     def clip_params(self):
         """Clamp the parameters to their ranges (``models.py:577-584``)."""
@@ -140,7 +152,7 @@ class StrokeField(nn.Module):
         if steps_per_stroke <= 1:
             raise AssertionError(f'Too few steps per stroke: {steps_per_stroke}')
         train_frac = cur_step / (max_step - steps_per_stroke)
-        prev = int(self.stroke_step.item())
+        prev = self.n_strokes()
         nxt = int(self.max_num_strokes * min(max(train_frac, 0.0), 1.0) ** self.step_power)
         nxt = min(max(nxt + self.init_num_strokes, prev), self.max_num_strokes)
         if self.shape_params.grad is not None:
@@ -189,6 +201,7 @@ class StrokeField(nn.Module):
                 self.density_params.data[i] = 1.0
                 off += 1
             self.stroke_step.fill_(nxt)
+            self._host_step = nxt
             self.last_update_step = cur_step
             updated = (prev, nxt, n_reset)
         self.clip_params()
@@ -196,7 +209,7 @@ class StrokeField(nn.Module):
 
     def stroke_range(self):
         """(first optimised stroke, stroke count) as predict_density truncates them (``models.py:666-673``)."""
-        step = int(self.stroke_step.item())
+        step = self.n_strokes()
         fixed = max(0, step - self.max_opt_strokes)
         if self.stroke_step_limit is not None:
             step = min(step, self.stroke_step_limit)
@@ -244,7 +257,11 @@ class StrokeField(nn.Module):
             sp = torch.cat([sp[:fixed].detach(), sp[fixed:]])
             cp = torch.cat([cp[:fixed].detach(), cp[fixed:]])
             dp = torch.cat([dp[:fixed].detach(), dp[fixed:]])
-        ids = torch.full((step,), self.sdf_id, dtype=torch.int32)
+        dev = self.shape_params.device
+        if self._ids is None or self._ids.device != dev:
+            self._ids = torch.full((self.max_num_strokes,), self.sdf_id, dtype=torch.int32, device=dev)
+        ids = self._ids[:step]
+        ids._n3ds_host_checked = True  # one known stroke type whose width is the parameters' width
         return origins, directions, viewdirs, radii, tdist, ids, sp, cp, dp
 
 

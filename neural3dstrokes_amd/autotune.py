@@ -8,7 +8,10 @@ Tunes, by timing the real workload on the device, the parameters exposed by
 * ``bwd_oversub``: backward persistent blocks per block resident at once;
 * ``batch_oversub``: a scene batch's backward blocks per resident block;
 * ``cull_group``: 64-stroke chunks culled by one wavefront of the culling pass
-  (a scene batch's plan depends on it: replan after changing it).
+  (a scene batch's plan depends on it: replan after changing it);
+* ``bwd_queue``: the adjoint queues its live (sample, stroke) pairs per SDF
+  family across rays and evaluates full groups of one family (1), or evaluates
+  each window's pairs as they come (0).
 
 Results do not depend on them beyond float summation order.  ``tune`` runs a
 caller-supplied step over a small grid (coordinate descent, one parameter at a
@@ -19,9 +22,9 @@ import torch
 
 from . import _lib
 
-PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3}
+PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3, 'bwd_queue': 4}
 SPACE = {'flush_pairs': (32, 48, 64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16),
-         'cull_group': (1, 2, 4, 8, 12, 16)}
+         'cull_group': (1, 2, 4, 8, 12, 16), 'bwd_queue': (0, 1)}
 _cache = {}
 
 

@@ -1619,13 +1619,143 @@ __global__ __launch_bounds__(64 * FWAVES) __attribute__((amdgpu_waves_per_eu(4, 
     render_fwd_body<NS, CID, true>(A0, rgb_out, acc_out, depth_out, state, counters);
 }
 
-template <int NS, int CID>
+// Parameter-gradient slots G (slot_column layout, colour from 14, density 14 + Q) of
+// one (sample, stroke) pair: stroke k, sample position x and radius r, view direction
+// vd, and the pair's upstream dL/dalpha, dL/d(density param) and dL/d(colour).
+template <int CID>
+__device__ __forceinline__ void pair_grad(const RenderArgs& A, int k, V3 x, float r, V3 vd, float ga, float gdw,
+                                          V3 gcol, float (&G)[15 + num_color_params(CID)]) {
+    constexpr int NQ = num_color_params(CID);
+    const float* rec = A.table + (int64_t)k * REC;
+    const int base = ibits(rec[R_BASE]);
+    const bool curve = ibits(rec[R_CURVE]) != 0;
+    const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
+    const float* cp = A.cparams + (int64_t)k * A.Q;
+    const float* prm = curve ? rec + R_CP : rec + R_BP;
+    if (curve && A.exact) {
+        // curves carry no transform: one closest-point solve gives the value (hence alpha
+        // and dL/dSDF) and the parameter adjoint
+        const ExactTransfer tr{ga, (A.laplace ? 2.f : 0.5f) / (A.delta * r), A.delta, A.laplace != 0};
+        float sdf, gs;
+        curve_value_grad(base, x, prm, tr, G, sdf, gs);
+        color_grad_t<CID>(x, vd, cp, gcol, G + 14, true);
+        G[14 + NQ] += gdw;
+        return;
+    }
+    const Pair q = eval_pair(rec, base, curve, x, r, A);
+    V3 gpos = color_grad_t<CID>(q.p, vd, cp, gcol, G + 14, A.exact != 0);
+    G[14 + NQ] += gdw;
+    const float gs = ga * dalpha_dsdf(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0, A.exact != 0);
+    const V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
+    gpos = gpos + dsdp * gs;
+    if (!curve) {
+        float gs1 = 0.f;
+        V3 gs3 = v3s(0.f), gang = v3s(0.f), gtr = v3s(0.f);
+        Frame F;
+        F.trans = ld3(rec + R_TR);
+        F.angle = ld3(rec + R_ANG);
+        F.sz = rec[R_SZ];
+        F.cz = rec[R_CZ];
+        F.scale = ld3(rec + R_SC);
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++)
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) F.R.r[ii][jj] = rec[R_ROT + 3 * ii + jj];
+        if (A.exact && A.isr && (flags & (F_SS | F_MS))) {
+            const float gsc = ga * dalpha_dscale(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0);
+            if (flags & F_SS) gs1 += gsc * q.sc / F.scale.x;
+            else radius_factor_adjoint(base, q.p, prm, F.scale, gsc * q.sc / q.fac, gpos, gs3, G);
+        }
+        xform_adjoint(gpos, q.p, x, F, flags, gs1, gs3, gang, gtr);
+        if (flags & F_SS) G[2] += gs1;
+        else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
+        G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
+        G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
+    }
+}
+
+// Adds the slots G of the first nl lanes to dst [S, PG], summed per run of lanes with
+// the same stroke key (runs are contiguous), through an LDS transpose: lane l stages
+// its slots, then each lane sums one (run, slot) segment and issues one atomic.
+template <int NG>
+__device__ __forceinline__ void reduce_runs(const RenderArgs& A, const WaveMem& w, int lane, int nl, int key,
+                                            const float (&G)[NG], float* dst, int PG) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
+    w.scr[lane * (NG + 1) + NG] = __int_as_float(key);
+    const int kprev = __shfl_up(key, 1, 64);
+    const bool head = lane < nl && (lane == 0 || key != kprev);
+    const unsigned long long hm = __ballot(head);
+    const int nseg = __popcll(hm);
+    if (head) w.xo[__popcll(hm & ((1ull << lane) - 1ull))] = lane;
+    wave_sync();
+    const int items = nseg * NG;
+    for (int it = lane; it < items; it += 64) {
+        const int jseg = it / NG;
+        const int g = it - jseg * NG;
+        const int lo = w.xo[jseg];
+        const int hi = jseg + 1 < nseg ? w.xo[jseg + 1] : nl;
+        const int kk = __float_as_int(w.scr[lo * (NG + 1) + NG]);
+        float sum = 0.f;
+        for (int l = lo; l < hi; l++) sum += w.scr[l * (NG + 1) + g];
+        if (sum != 0.f) {
+            const int col = slot_column(g, A.table + (int64_t)kk * REC, A.Pmax, A.Q);
+            if (col >= 0) atomicAdd(&dst[(int64_t)kk * PG + col], sum);
+        }
+    }
+    wave_sync();
+}
+
+// Pending live pairs of the adjoint, queued per wavefront and class in global memory
+// so that the gradient groups run 64 pairs of one SDF family at a time, across rays.
+// Field-major ring of RCAP slots per class: stroke, ray, sample depth, dL/dalpha,
+// dL/d(density param), dL/d(colour) x 3.
+constexpr int RCAP = 320;
+constexpr int RW = 8;
+static_assert(RCAP >= 63 + PBUF, "a window's live pairs of one class fit behind fewer than 64 pending ones");
+__host__ __device__ constexpr int64_t ring_floats_per_wave() { return (int64_t)NCLASS * RCAP * RW; }
+
+// global-memory ordering between the lanes of one wavefront (the queue's writes, then
+// reads by other lanes)
+__device__ __forceinline__ void queue_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Evaluates nl (<= 64) queued pairs of one class from slot head on.
+template <int CID>
+__device__ __forceinline__ void queued_group(const RenderArgs& A, const WaveMem& w, int lane, const float* rq,
+                                             int head, int nl, float* dst, int PG) {
+    constexpr int NG = 15 + num_color_params(CID);
+    float G[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) G[g] = 0.f;
+    int k = 0x7fffffff;  // sorts after every stroke
+    if (lane < nl) {
+        const int slot = (head + lane) % RCAP;
+        k = __float_as_int(rq[slot]);
+        const int64_t ray = __float_as_int(rq[RCAP + slot]);
+        const float t = rq[2 * RCAP + slot];
+        const V3 o = ld3(A.origins + 3 * ray), d = ld3(A.dirs + 3 * ray), vd = ld3(A.viewdirs + 3 * ray);
+        V3 x = d * t + o;
+        if (A.coord_div != 1.f) x = v3(x.x / A.coord_div, x.y / A.coord_div, x.z / A.coord_div);
+        const float r = A.cone ? A.radii[ray] * t : A.radii[ray];
+        pair_grad<CID>(A, k, x, r, vd, rq[3 * RCAP + slot], rq[4 * RCAP + slot],
+                       v3(rq[5 * RCAP + slot], rq[6 * RCAP + slot], rq[7 * RCAP + slot]), G);
+    }
+    reduce_runs<NG>(A, w, lane, nl, k, G, dst, PG);
+}
+
+// QUEUE: the pending-pair queues (queues != nullptr); without, gradient groups run window by window
+template <int NS, int CID, bool QUEUE>
 __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, const float* __restrict__ g_rgb,
                                                                  const float* __restrict__ g_acc,
                                                                  const float* __restrict__ g_depth,
                                                                  const float* __restrict__ state,
                                                                  float* __restrict__ partials, int use_lds,
-                                                                 float* __restrict__ gdirect) {
+                                                                 float* __restrict__ gdirect,
+                                                                 float* __restrict__ queues) {
     extern __shared__ float smem[];  // [S * PG] accumulators (use_lds) + per-wavefront regions
     constexpr int NQ = num_color_params(CID);
     constexpr int NG = 15 + NQ;  // gradient slots per pair
@@ -1639,6 +1769,9 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     int64_t ray0 = (int64_t)blockIdx.x * W + wave, ray_end = A0.R, ray_step = (int64_t)gridDim.x * W;
     float* dst = use_lds ? smem : gdirect;
     float* prow = partials ? partials + (int64_t)blockIdx.x * nacc_max : nullptr;
+    // this wavefront's pending-pair queues (nullptr: gradient groups run window by window)
+    float* queue = QUEUE ? queues + ((int64_t)blockIdx.x * W + wave) * ring_floats_per_wave() : nullptr;
+    int qhead[NCLASS] = {0, 0, 0}, qtail[NCLASS] = {0, 0, 0};
     if (A0.scenes) {  // a batch: this block works on one scene's rays
         const int b = A0.block_base + (int)blockIdx.x;
         const int sc = A0.scenes[A0.n_scenes * SCENE_REC + b];
@@ -1991,107 +2124,143 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 }
                 wave_sync();
                 const int nu = xs[NCLASS];
-                for (int u0 = 0; u0 < nu; u0 += 64) {
-                    const int u = u0 + lane;
-                    float G[NG];
+                if constexpr (QUEUE) {
+                    // queue the live pairs class by class behind the pending ones; a class's
+                    // full groups of 64 are evaluated as soon as they are there
 #pragma unroll
-                    for (int g = 0; g < NG; g++) G[g] = 0.f;
-                    int c = 0x7fffffff;  // sorts after every hit index
-                    if (u < nu) {
-                        const int p = w.ul[u] & 0xffff;
-                        c = w.ul[u] >> 16;
-                        const float ga = w.pb[p];
-                        const float gdw = w.pb[PBUF + p];
-                        const V3 gcol = v3(w.pb[2 * PBUF + p], w.pb[3 * PBUF + p], w.pb[4 * PBUF + p]);
-                        {
-                            const int k = w.ck[c];
-                            const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
-                            V3 x;
-                            float r;
-                            sample_point(A, rc, w.ts[i], x, r);
-                            const float* rec = A.table + (int64_t)k * REC;
-                            const int base = ibits(rec[R_BASE]);
-                            const bool curve = ibits(rec[R_CURVE]) != 0;
-                            const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
-                            const float* cp = A.cparams + (int64_t)k * A.Q;
-                            const float* prm = curve ? rec + R_CP : rec + R_BP;
-                            if (curve && A.exact) {
-                                // curves carry no transform: one closest-point solve gives the
-                                // value (hence alpha and dL/dSDF) and the parameter adjoint
-                                const ExactTransfer tr{ga, (A.laplace ? 2.f : 0.5f) / (A.delta * r), A.delta,
-                                                       A.laplace != 0};
-                                float sdf, gs;
-                                curve_value_grad(base, x, prm, tr, G, sdf, gs);
-                                color_grad_t<CID>(x, rc.vd, cp, gcol, G + 14, true);
+                    for (int xc = 0; xc < NCLASS; xc++) {
+                        const int nx = xs[xc + 1] - xs[xc];
+                        float* rq = queue + xc * (RCAP * RW);
+                        for (int u = lane; u < nx; u += 64) {
+                            const int e = w.ul[xs[xc] + u];
+                            const int p = e & 0xffff, c = e >> 16;
+                            const int slot = (qtail[xc] + u) % RCAP;
+                            rq[slot] = __int_as_float(w.ck[c]);
+                            rq[RCAP + slot] = __int_as_float((int)ray);
+                            rq[2 * RCAP + slot] = w.ts[w.ci0[c] + (p - (w.coff[c] - wbase))];
+                            rq[3 * RCAP + slot] = w.pb[p];
+                            rq[4 * RCAP + slot] = w.pb[PBUF + p];
+                            rq[5 * RCAP + slot] = w.pb[2 * PBUF + p];
+                            rq[6 * RCAP + slot] = w.pb[3 * PBUF + p];
+                            rq[7 * RCAP + slot] = w.pb[4 * PBUF + p];
+                        }
+                        qtail[xc] += nx;
+                    }
+                    queue_sync();
+#pragma unroll
+                    for (int xc = 0; xc < NCLASS; xc++)
+                        for (; qtail[xc] - qhead[xc] >= 64; qhead[xc] += 64)
+                            queued_group<CID>(A, w, lane, queue + xc * (RCAP * RW), qhead[xc] % RCAP, 64, dst, PG);
+                } else {
+                    for (int u0 = 0; u0 < nu; u0 += 64) {
+                        const int u = u0 + lane;
+                        float G[NG];
+    #pragma unroll
+                        for (int g = 0; g < NG; g++) G[g] = 0.f;
+                        int c = 0x7fffffff;  // sorts after every hit index
+                        if (u < nu) {
+                            const int p = w.ul[u] & 0xffff;
+                            c = w.ul[u] >> 16;
+                            const float ga = w.pb[p];
+                            const float gdw = w.pb[PBUF + p];
+                            const V3 gcol = v3(w.pb[2 * PBUF + p], w.pb[3 * PBUF + p], w.pb[4 * PBUF + p]);
+                            {
+                                const int k = w.ck[c];
+                                const int i = w.ci0[c] + (p - (w.coff[c] - wbase));
+                                V3 x;
+                                float r;
+                                sample_point(A, rc, w.ts[i], x, r);
+                                const float* rec = A.table + (int64_t)k * REC;
+                                const int base = ibits(rec[R_BASE]);
+                                const bool curve = ibits(rec[R_CURVE]) != 0;
+                                const uint32_t flags = (uint32_t)ibits(rec[R_FLAGS]);
+                                const float* cp = A.cparams + (int64_t)k * A.Q;
+                                const float* prm = curve ? rec + R_CP : rec + R_BP;
+                                if (curve && A.exact) {
+                                    // curves carry no transform: one closest-point solve gives the
+                                    // value (hence alpha and dL/dSDF) and the parameter adjoint
+                                    const ExactTransfer tr{ga, (A.laplace ? 2.f : 0.5f) / (A.delta * r), A.delta,
+                                                           A.laplace != 0};
+                                    float sdf, gs;
+                                    curve_value_grad(base, x, prm, tr, G, sdf, gs);
+                                    color_grad_t<CID>(x, rc.vd, cp, gcol, G + 14, true);
+                                    G[14 + NQ] += gdw;
+                                } else {
+                                const Pair q = eval_pair(rec, base, curve, x, r, A);
+                                V3 gpos = color_grad_t<CID>(q.p, rc.vd, cp, gcol, G + 14, A.exact != 0);
                                 G[14 + NQ] += gdw;
-                            } else {
-                            const Pair q = eval_pair(rec, base, curve, x, r, A);
-                            V3 gpos = color_grad_t<CID>(q.p, rc.vd, cp, gcol, G + 14, A.exact != 0);
-                            G[14 + NQ] += gdw;
-                            const float gs = ga * dalpha_dsdf(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0,
-                                                              A.exact != 0);
-                            const V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
-                            gpos = gpos + dsdp * gs;
-                            if (!curve) {
-                                float gs1 = 0.f;
-                                V3 gs3 = v3s(0.f), gang = v3s(0.f), gtr = v3s(0.f);
-                                Frame F;
-                                F.trans = ld3(rec + R_TR);
-                                F.angle = ld3(rec + R_ANG);
-                                F.sz = rec[R_SZ];
-                                F.cz = rec[R_CZ];
-                                F.scale = ld3(rec + R_SC);
-#pragma unroll
-                                for (int ii = 0; ii < 3; ii++)
-#pragma unroll
-                                    for (int jj = 0; jj < 3; jj++) F.R.r[ii][jj] = rec[R_ROT + 3 * ii + jj];
-                                if (A.exact && A.isr && (flags & (F_SS | F_MS))) {
-                                    const float gsc = ga * dalpha_dscale(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0);
-                                    if (flags & F_SS) gs1 += gsc * q.sc / F.scale.x;
-                                    else radius_factor_adjoint(base, q.p, prm, F.scale, gsc * q.sc / q.fac, gpos, gs3, G);
+                                const float gs = ga * dalpha_dsdf(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0,
+                                                                  A.exact != 0);
+                                const V3 dsdp = A.exact ? grad_ex(base, q.p, prm, gs, G) : grad_ref(base, q.p, prm, gs, G);
+                                gpos = gpos + dsdp * gs;
+                                if (!curve) {
+                                    float gs1 = 0.f;
+                                    V3 gs3 = v3s(0.f), gang = v3s(0.f), gtr = v3s(0.f);
+                                    Frame F;
+                                    F.trans = ld3(rec + R_TR);
+                                    F.angle = ld3(rec + R_ANG);
+                                    F.sz = rec[R_SZ];
+                                    F.cz = rec[R_CZ];
+                                    F.scale = ld3(rec + R_SC);
+    #pragma unroll
+                                    for (int ii = 0; ii < 3; ii++)
+    #pragma unroll
+                                        for (int jj = 0; jj < 3; jj++) F.R.r[ii][jj] = rec[R_ROT + 3 * ii + jj];
+                                    if (A.exact && A.isr && (flags & (F_SS | F_MS))) {
+                                        const float gsc = ga * dalpha_dscale(q.sdf, q.alpha, q.sc, A.delta, A.laplace != 0);
+                                        if (flags & F_SS) gs1 += gsc * q.sc / F.scale.x;
+                                        else radius_factor_adjoint(base, q.p, prm, F.scale, gsc * q.sc / q.fac, gpos, gs3, G);
+                                    }
+                                    xform_adjoint(gpos, q.p, x, F, flags, gs1, gs3, gang, gtr);
+                                    if (flags & F_SS) G[2] += gs1;
+                                    else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
+                                    G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
+                                    G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
                                 }
-                                xform_adjoint(gpos, q.p, x, F, flags, gs1, gs3, gang, gtr);
-                                if (flags & F_SS) G[2] += gs1;
-                                else { G[2] += gs3.x; G[3] += gs3.y; G[4] += gs3.z; }
-                                G[5] += gang.x; G[6] += gang.y; G[7] += gang.z;
-                                G[8] += gtr.x; G[9] += gtr.y; G[10] += gtr.z;
-                            }
+                                }
                             }
                         }
-                    }
-                    // segmented sum over the lanes of each hit through an LDS transpose (the
-                    // lanes of one hit are contiguous; w.xo lists the first lane of each run)
-                    const int nl = min(64, nu - u0);
-#pragma unroll
-                    for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
-                    w.scr[lane * (NG + 1) + NG] = __int_as_float(c);
-                    const int cprev = __shfl_up(c, 1, 64);
-                    const bool head = lane < nl && (lane == 0 || c != cprev);
-                    const unsigned long long hm = __ballot(head);
-                    const int nseg = __popcll(hm);
-                    if (head) w.xo[__popcll(hm & ((1ull << lane) - 1ull))] = lane;
-                    wave_sync();
-                    const int items = nseg * NG;
-                    for (int it = lane; it < items; it += 64) {
-                        const int jseg = it / NG;
-                        const int g = it - jseg * NG;
-                        const int lo = w.xo[jseg];
-                        const int hi = jseg + 1 < nseg ? w.xo[jseg + 1] : nl;
-                        const int cc = __float_as_int(w.scr[lo * (NG + 1) + NG]);
-                        float sum = 0.f;
-                        for (int l = lo; l < hi; l++) sum += w.scr[l * (NG + 1) + g];
-                        if (sum != 0.f) {
-                            const int k = w.ck[cc];
-                            const int col = slot_column(g, A.table + (int64_t)k * REC, A.Pmax, A.Q);
-                            if (col >= 0) atomicAdd(&dst[(int64_t)k * PG + col], sum);
+                        // segmented sum over the lanes of each hit through an LDS transpose (the
+                        // lanes of one hit are contiguous; w.xo lists the first lane of each run)
+                        const int nl = min(64, nu - u0);
+    #pragma unroll
+                        for (int g = 0; g < NG; g++) w.scr[lane * (NG + 1) + g] = G[g];
+                        w.scr[lane * (NG + 1) + NG] = __int_as_float(c);
+                        const int cprev = __shfl_up(c, 1, 64);
+                        const bool head = lane < nl && (lane == 0 || c != cprev);
+                        const unsigned long long hm = __ballot(head);
+                        const int nseg = __popcll(hm);
+                        if (head) w.xo[__popcll(hm & ((1ull << lane) - 1ull))] = lane;
+                        wave_sync();
+                        const int items = nseg * NG;
+                        for (int it = lane; it < items; it += 64) {
+                            const int jseg = it / NG;
+                            const int g = it - jseg * NG;
+                            const int lo = w.xo[jseg];
+                            const int hi = jseg + 1 < nseg ? w.xo[jseg + 1] : nl;
+                            const int cc = __float_as_int(w.scr[lo * (NG + 1) + NG]);
+                            float sum = 0.f;
+                            for (int l = lo; l < hi; l++) sum += w.scr[l * (NG + 1) + g];
+                            if (sum != 0.f) {
+                                const int k = w.ck[cc];
+                                const int col = slot_column(g, A.table + (int64_t)k * REC, A.Pmax, A.Q);
+                                if (col >= 0) atomicAdd(&dst[(int64_t)k * PG + col], sum);
+                            }
                         }
+                        wave_sync();
                     }
-                    wave_sync();
                 }
                 c0 = c1;
             }
             nh = np = 0;
         }
+    }
+    if constexpr (QUEUE) {  // the partial groups left in the queues
+#pragma unroll
+        for (int xc = 0; xc < NCLASS; xc++)
+            for (; qtail[xc] > qhead[xc]; qhead[xc] += 64)
+                queued_group<CID>(A, w, lane, queue + xc * (RCAP * RW), qhead[xc] % RCAP,
+                                  min(64, qtail[xc] - qhead[xc]), dst, PG);
     }
     if (use_lds) {
         __syncthreads();
@@ -2165,13 +2334,19 @@ static int g_max_lds = 0;
 // Launch-shape parameters chosen by the autotuner (n3ds_render_tuning): pending pairs
 // that trigger an evaluation window, backward persistent blocks per resident block,
 // the batch backward's blocks per resident block, and chunks per culling task.
-enum { TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_CULL_GROUP = 3, TUNE_COUNT = 4 };
-static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1};
-static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1};
-static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16};
+// TUNE_BWD_QUEUE: the adjoint queues live pairs per SDF family across rays (1) or
+// evaluates each window's pairs as they come (0)
+enum {
+    TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_CULL_GROUP = 3, TUNE_BWD_QUEUE = 4,
+    TUNE_COUNT = 5
+};
+static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1, 1};
+static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1, 0};
+static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16, 1};
 
 typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
-typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*);
+typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*,
+                          float*);
 
 static FwdKernel fwd_pre_kernel_ptr(int ns, int cid) {
     static const FwdKernel tab[4][4] = {
@@ -2195,13 +2370,13 @@ static FwdKernel fwd_kernel_ptr(int ns, int cid) {
     return tab[ns - 1][cid];
 }
 
-static BwdKernel bwd_kernel_ptr(int ns, int cid) {
-    static const BwdKernel tab[4][4] = {
-        {render_bwd_kernel<1, 0>, render_bwd_kernel<1, 1>, render_bwd_kernel<1, 2>, render_bwd_kernel<1, 3>},
-        {render_bwd_kernel<2, 0>, render_bwd_kernel<2, 1>, render_bwd_kernel<2, 2>, render_bwd_kernel<2, 3>},
-        {render_bwd_kernel<3, 0>, render_bwd_kernel<3, 1>, render_bwd_kernel<3, 2>, render_bwd_kernel<3, 3>},
-        {render_bwd_kernel<4, 0>, render_bwd_kernel<4, 1>, render_bwd_kernel<4, 2>, render_bwd_kernel<4, 3>}};
-    return tab[ns - 1][cid];
+#define N3DS_BWD_ROW(NS, Q) \
+    {render_bwd_kernel<NS, 0, Q>, render_bwd_kernel<NS, 1, Q>, render_bwd_kernel<NS, 2, Q>, render_bwd_kernel<NS, 3, Q>}
+static BwdKernel bwd_kernel_ptr(int ns, int cid, bool queue = false) {
+    static const BwdKernel tab[2][4][4] = {
+        {N3DS_BWD_ROW(1, false), N3DS_BWD_ROW(2, false), N3DS_BWD_ROW(3, false), N3DS_BWD_ROW(4, false)},
+        {N3DS_BWD_ROW(1, true), N3DS_BWD_ROW(2, true), N3DS_BWD_ROW(3, true), N3DS_BWD_ROW(4, true)}};
+    return tab[queue ? 1 : 0][ns - 1][cid];
 }
 
 static int fwd_lds_bytes(int ns, int cid) { return FWAVES * wave_mem_floats(ns, cid == 1 ? 4 : 1, 0) * 4; }
@@ -2246,6 +2421,27 @@ static int device_info() {
         N3DS_HIP_CHECK(hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, dev));
         N3DS_HIP_CHECK(hipDeviceGetAttribute(&g_max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     }
+    return N3DS_OK;
+}
+
+typedef void (*BwdKernelT)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*,
+                           float*);
+
+// Persistent backward blocks for n_rays rays: as many as fit on the device at once
+// (registers and LDS) times the oversubscription setting, at most one per bs.waves rays.
+static int bwd_blocks(BwdKernelT kern, const BwdShape& bs, int64_t n_rays, int64_t* nb_out) {
+    if (bs.lds_bytes > 65536)
+        N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bs.lds_bytes));
+    int64_t nb = max_persistent_blocks(n_rays);
+    const int64_t need = (n_rays + bs.waves - 1) / bs.waves;
+    if (nb > need) nb = need;
+    int per_cu = 0;
+    N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * bs.waves, (size_t)bs.lds_bytes));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t fit = (int64_t)g_num_cu * per_cu * g_tune[TUNE_BWD_OVERSUB];
+    if (fit < nb) nb = fit;
+    *nb_out = nb < 1 ? 1 : nb;
     return N3DS_OK;
 }
 
@@ -2422,10 +2618,20 @@ N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_
     int cid = 0;
     for (int c = 0; c < 4; c++)
         if (num_color_params(c) == color_width) cid = c;
-    bool lds = false;
-    for (int ns = 1; ns <= 4; ns++) lds = lds || bwd_shape(ns, cid, n_strokes, (int)PG).use_lds;
-    if (!lds) return (int64_t)n_strokes * PG;  // global-atomic path
-    return max_persistent_blocks(n_rays) * n_strokes * PG;
+    // the largest over the sample-count variants: gradient rows (per persistent block when
+    // they fit in LDS, else one global set) and the pending-pair queues
+    const int64_t tot = (int64_t)n_strokes * PG;
+    int64_t need = tot;
+    for (int ns = 1; ns <= 4; ns++) {
+        const BwdShape bs = bwd_shape(ns, cid, n_strokes, (int)PG);
+        if (bs.lds_bytes > g_max_lds) continue;
+        int64_t nb = 0;
+        if (bwd_blocks(bwd_kernel_ptr(ns, cid, g_tune[TUNE_BWD_QUEUE] != 0), bs, n_rays, &nb)) return -1;
+        int64_t n = bs.use_lds ? nb * tot : tot;
+        if (g_tune[TUNE_BWD_QUEUE]) n += nb * bs.waves * ring_floats_per_wave();
+        if (n > need) need = n;
+    }
+    return need;
 }
 
 // Backward body shared by n3ds_render_backward (caller workspace) and
@@ -2459,32 +2665,27 @@ static int render_backward_impl(const float* origins, const float* directions, c
     const int ns = (n_samples + 63) / 64;
     const BwdShape bs = bwd_shape(ns, color_id, n_strokes, PG);
     N3DS_REQUIRE(bs.lds_bytes <= g_max_lds, "backward needs %lld bytes of LDS per block", (long long)bs.lds_bytes);
-    BwdKernel kern = bwd_kernel_ptr(ns, color_id);
-    if (bs.lds_bytes > 65536)
-        N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)bs.lds_bytes));
-    // as many persistent blocks as fit on the device at once (registers and LDS)
-    int64_t nb = max_persistent_blocks(n_rays);
-    {
-        const int64_t need = (n_rays + bs.waves - 1) / bs.waves;
-        if (nb > need) nb = need;
-        int per_cu = 0;
-        N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * bs.waves,
-                                                                    (size_t)bs.lds_bytes));
-        if (per_cu < 1) per_cu = 1;
-        const int64_t fit = (int64_t)g_num_cu * per_cu * g_tune[TUNE_BWD_OVERSUB];
-        if (fit < nb) nb = fit;
-    }
     const int64_t tot = (int64_t)n_strokes * PG;
-    if (fit_blocks) {
+    if (fit_blocks)
         N3DS_REQUIRE(tot <= workspace_floats,
                      "%d strokes x %d gradient columns exceed the %lld-float scratch of the allocation-free backward",
                      n_strokes, PG, (long long)workspace_floats);
-        if (bs.use_lds && nb * tot > workspace_floats) nb = workspace_floats / tot;
+    // the pending-pair queues follow the gradient rows when the workspace has room for them
+    bool queue = !fit_blocks && g_tune[TUNE_BWD_QUEUE] != 0;
+    BwdKernel kern = nullptr;
+    int64_t nb = 0, ws_need = 0;
+    for (;;) {
+        kern = bwd_kernel_ptr(ns, color_id, queue);
+        st = bwd_blocks(kern, bs, n_rays, &nb);
+        if (st) return st;
+        if (fit_blocks && bs.use_lds && nb * tot > workspace_floats) nb = workspace_floats / tot;
+        ws_need = bs.use_lds ? nb * tot : tot;
+        if (!queue || workspace_floats >= ws_need + nb * bs.waves * ring_floats_per_wave()) break;
+        queue = false;
     }
-    const int64_t ws_need = bs.use_lds ? nb * tot : tot;
     N3DS_REQUIRE(workspace_floats >= ws_need, "workspace has %lld floats, need %lld", (long long)workspace_floats,
                  (long long)ws_need);
+    float* queues = queue ? workspace + ws_need : nullptr;
     hipStream_t s = (hipStream_t)stream;
     if (!bs.use_lds) {
         hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace, tot);
@@ -2493,7 +2694,7 @@ static int render_backward_impl(const float* origins, const float* directions, c
     float* parts = bs.use_lds ? workspace : nullptr;
     float* direct = bs.use_lds ? nullptr : workspace;
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * bs.waves), (size_t)bs.lds_bytes, s, A, grad_rgb, grad_acc,
-                       grad_depth, state, parts, bs.use_lds, direct);
+                       grad_depth, state, parts, bs.use_lds, direct, queues);
     N3DS_LAUNCH_CHECK();
     const int nparts = bs.use_lds ? (int)nb : 1;
     hipLaunchKernelGGL(render_reduce_kernel,
@@ -2857,7 +3058,7 @@ N3DS_EXPORT int n3ds_render_batch_backward(const float* origins, const float* di
                                                (int)bs.lds_bytes));
         hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(64 * bs.waves), (size_t)bs.lds_bytes, s, A,
                            grad_rgb, grad_acc, grad_depth, state, bs.use_lds ? parts : nullptr, bs.use_lds,
-                           bs.use_lds ? nullptr : direct);
+                           bs.use_lds ? nullptr : direct, nullptr);
         N3DS_LAUNCH_CHECK();
     }
     const int64_t tot = (int64_t)n_strokes * PG;

@@ -11,7 +11,10 @@ Tunes, by timing the real workload on the device, the parameters exposed by
   (a scene batch's plan depends on it: replan after changing it);
 * ``bwd_queue``: the adjoint queues its live (sample, stroke) pairs per SDF
   family across rays and evaluates full groups of one family (1), or evaluates
-  each window's pairs as they come (0).
+  each window's pairs as they come (0);
+* ``cull_grid``: the culling pass of one scene tests each ray against the
+  strokes binned in the grid cells its segment crosses (1), or against every
+  stroke (0); the hits are the same.
 
 Results do not depend on them beyond float summation order.  ``tune`` runs a
 caller-supplied step over a small grid (coordinate descent, one parameter at a
@@ -22,9 +25,9 @@ import torch
 
 from . import _lib
 
-PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3, 'bwd_queue': 4}
+PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3, 'bwd_queue': 4, 'cull_grid': 5}
 SPACE = {'flush_pairs': (32, 48, 64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16),
-         'cull_group': (1, 2, 4, 8, 12, 16), 'bwd_queue': (0, 1)}
+         'cull_group': (1, 2, 4, 8, 12, 16), 'bwd_queue': (0, 1), 'cull_grid': (0, 1)}
 _cache = {}
 
 

@@ -1167,6 +1167,620 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
 }
 
 // ---------------------------------------------------------------------------
+// Grid-assisted culling (one scene of at most GRID_MAX_NCH chunks).  A uniform
+// grid over the strokes' culling bounds holds, per cell and 64-stroke chunk, the
+// mask of the strokes whose bound (for a curve: one of its pieces' chord boxes),
+// grown by the largest ray margin, overlaps the cell.  Every ray ORs the masks of
+// the cells its segment crosses.  A stroke outside its ray's mask fails the
+// culling tests of render_cull_kernel for that ray, so render_cull_grid_kernel
+// runs those tests on the (ray, stroke) pairs of the mask only, 64 to a
+// wavefront, and writes the same hits.
+// ---------------------------------------------------------------------------
+constexpr int GRID_MAX_NCH = 64;
+constexpr int64_t GRID_WORDS = 1 << 19;  // cell-mask words (64-bit) at most, 4 MiB
+constexpr int GRID_RES_MAX = 32;
+constexpr int GRID_RAY_WAVES = 4;        // wavefronts per block of grid_rays_kernel
+constexpr int GRID_RAYS_PER_WAVE = 4;    // rays per wavefront of grid_rays_kernel
+constexpr int GRID_SETUP_THREADS = 1024;
+constexpr int GRID_CAP = 128;            // candidates per culling batch
+constexpr int GRID_STRIDE = 64;          // chunks per culling task at most (one task per ray)
+// parameter block: box low corner [3], cell size [3], inverse cell size [3],
+// resolution, all-candidates flag, largest ray margin, cell inflation
+enum { GP_LO = 0, GP_CELL = 3, GP_ICELL = 6, GP_RES = 9, GP_ALL = 10, GP_M = 11, GP_DELTA = 12, GP_COUNT = 16 };
+
+static int grid_res(int nch) {
+    int r = GRID_RES_MAX;
+    while (r > 4 && (int64_t)r * r * r * nch > GRID_WORDS) r--;
+    return r;
+}
+
+__device__ __forceinline__ bool finite3(V3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+// Axis-aligned bound of the region in which stroke k (piece `piece` of a curve) can
+// pass the culling tests of a ray whose margin is at most m: the unit-space box of
+// cull_stroke mapped back by R diag(s), or the chord box of the piece test.
+// Returns false for a solid's piece > 0.
+__device__ __forceinline__ bool grid_item_box(const float* __restrict__ rec, int piece, float m, int isr, V3& lo,
+                                              V3& hi) {
+    if (ibits(rec[R_CURVE]) != 0) {
+        const float e = rec[R_RMAX] + m + rec[R_DEV + piece];
+        const V3 a = ld3(rec + R_PTS + 3 * piece), b = ld3(rec + R_PTS + 3 * piece + 3);
+        lo = v3(fminf(a.x, b.x) - e, fminf(a.y, b.y) - e, fminf(a.z, b.z) - e);
+        hi = v3(fmaxf(a.x, b.x) + e, fmaxf(a.y, b.y) + e, fmaxf(a.z, b.z) + e);
+        return true;
+    }
+    if (piece) return false;
+    const float mu = isr ? m / rec[R_MINS] : m;
+    float e[3], h[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) e[a] = (rec[R_E0 + a] + rec[R_E1 + a] * mu) * rec[R_SC + a];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        h[i] = fabsf(rec[R_ROT + 3 * i]) * e[0] + fabsf(rec[R_ROT + 3 * i + 1]) * e[1] +
+               fabsf(rec[R_ROT + 3 * i + 2]) * e[2];
+    const V3 tr = ld3(rec + R_TR);
+    lo = v3(tr.x - h[0], tr.y - h[1], tr.z - h[2]);
+    hi = v3(tr.x + h[0], tr.y + h[1], tr.z + h[2]);
+    return true;
+}
+
+// Per ray: the segment [tlo, thi] and margin m that the culling pass derives from the
+// sample depths (load_ray, or unsorted_range when the depths are not sorted), and the
+// block's largest margin; the whole launch also clears the cell masks.  A wavefront
+// takes GRID_RAYS_PER_WAVE rays, their depths loaded together.
+
+template <int NS>
+__global__ __launch_bounds__(64 * GRID_RAY_WAVES) void grid_rays_kernel(RenderArgs A, float* __restrict__ seg,
+                                                                         float* __restrict__ part,
+                                                                         unsigned long long* __restrict__ cells,
+                                                                         int64_t ncell_words) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, gn = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = gt; i < ncell_words; i += gn) cells[i] = 0ull;
+    __shared__ float bm[GRID_RAY_WAVES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ray0 = ((int64_t)blockIdx.x * GRID_RAY_WAVES + wave) * GRID_RAYS_PER_WAVE;
+    float tm[GRID_RAYS_PER_WAVE][NS], tn[GRID_RAYS_PER_WAVE][NS], rad[GRID_RAYS_PER_WAVE];
+#pragma unroll
+    for (int r = 0; r < GRID_RAYS_PER_WAVE; r++) {
+        const int64_t ray = ray0 + r;
+        rad[r] = A.radii[ray < A.R ? ray : 0];
+        const float* td = A.tdist + (ray < A.R ? ray : 0) * (A.n + 1);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int i = lane + 64 * s;
+            const bool ok = ray < A.R && i < A.n;
+            tm[r][s] = ok ? (td[i] + td[i + 1]) / 2.f : 0.f;
+            tn[r][s] = ok && i + 1 < A.n ? (td[i + 1] + td[i + 2]) / 2.f : 0.f;
+        }
+    }
+    float mmax = 0.f;
+#pragma unroll
+    for (int r = 0; r < GRID_RAYS_PER_WAVE; r++) {
+        const int64_t ray = ray0 + r;
+        if (ray >= A.R) break;
+        float lo = INFINITY, hi = -INFINITY;
+        bool bad = false;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int i = lane + 64 * s;
+            if (i < A.n) {
+                lo = fminf(lo, tm[r][s]);
+                hi = fmaxf(hi, tm[r][s]);
+                if (i + 1 < A.n && tm[r][s] > tn[r][s]) bad = true;
+            }
+        }
+        float tlo, thi;
+        if (__ballot(bad) == 0ull) {
+            // the first and last midpoints, (td[i] + td[i + 1]) / 2 as 0.5 (td[i] + td[i + 1])
+            tlo = __shfl(tm[r][0], 0, 64);
+            const int il = A.n - 1;
+            thi = 0.f;
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+                if (s == il / 64) thi = __shfl(tm[r][s], il % 64, 64);
+        } else {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                lo = fminf(lo, __shfl_xor(lo, off, 64));
+                hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+            }
+            tlo = lo;
+            thi = hi;
+        }
+        const float radii = rad[r];
+        const float rmax = A.cone ? radii * fmaxf(fabsf(tlo), fabsf(thi)) : radii;
+        const float m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
+        mmax = fmaxf(mmax, m);
+        if (lane == 0) {
+            seg[ray * 3] = tlo;
+            seg[ray * 3 + 1] = thi;
+            seg[ray * 3 + 2] = m;
+        }
+    }
+    if (lane == 0) bm[wave] = mmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = 0.f;
+        for (int q = 0; q < GRID_RAY_WAVES; q++) b = fmaxf(b, bm[q]);
+        part[blockIdx.x] = b;
+    }
+}
+
+// One block: the largest ray margin, the box of every stroke's (piece's) bound grown
+// by it, and the grid over their union (an all-candidates flag when a bound or a
+// margin is not finite).
+__global__ __launch_bounds__(GRID_SETUP_THREADS) void grid_setup_kernel(RenderArgs A, const float* __restrict__ part,
+                                                                        int nparts, int res, float* __restrict__ gp) {
+    __shared__ float red[GRID_SETUP_THREADS / 64][8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float m = 0.f;
+    bool bad = false;
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nparts; i += GRID_SETUP_THREADS) {
+        const float v = part[i];
+        if (!isfinite(v)) bad = true;
+        m = fmaxf(m, v);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane == 0) red[wave][0] = m;
+    __syncthreads();
+    m = 0.f;
+    for (int q = 0; q < GRID_SETUP_THREADS / 64; q++) m = fmaxf(m, red[q][0]);
+    __syncthreads();
+    V3 lo = v3s(INFINITY), hi = v3s(-INFINITY);
+    for (int k = threadIdx.x; k < A.S; k += GRID_SETUP_THREADS) {
+        const float* rec = A.table + (int64_t)k * REC;
+        const int np = ibits(rec[R_CURVE]) != 0 ? NPIECE : 1;
+#pragma unroll
+        for (int pc = 0; pc < NPIECE; pc++) {
+            if (pc < np) {
+                V3 a, b;
+                grid_item_box(rec, pc, m, A.isr, a, b);
+                if (!finite3(a) || !finite3(b)) bad = true;
+                lo = v3(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z));
+                hi = v3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
+            }
+        }
+    }
+    float v[6] = {lo.x, lo.y, lo.z, -hi.x, -hi.y, -hi.z};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        float x = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x = fminf(x, __shfl_xor(x, off, 64));
+        v[c] = x;
+    }
+    // the flag is a max: reduce its negation as a min
+    float f = bad ? -1.f : 0.f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f = fminf(f, __shfl_xor(f, off, 64));
+    if (lane == 0) {
+        for (int c = 0; c < 6; c++) red[wave][c] = v[c];
+        red[wave][6] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[7] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, 0.f};
+        for (int q = 0; q < GRID_SETUP_THREADS / 64; q++) {
+            for (int c = 0; c < 6; c++) r[c] = fminf(r[c], red[q][c]);
+            r[6] = fminf(r[6], red[q][6]);
+        }
+        const V3 blo = v3(r[0], r[1], r[2]), bhi = v3(-r[3], -r[4], -r[5]);
+        bool all = r[6] < 0.f || !isfinite(m) || !finite3(blo) || !finite3(bhi);
+        const float ext = fmaxf(bhi.x - blo.x, fmaxf(bhi.y - blo.y, bhi.z - blo.z));
+        const float mag = fmaxf(fmaxf(fabsf(blo.x), fabsf(blo.y)), fmaxf(fmaxf(fabsf(blo.z), fabsf(bhi.x)),
+                                                                     fmaxf(fabsf(bhi.y), fabsf(bhi.z))));
+        // cells are grown by 1 % of a cell (and a relative 1e-6) against rounding in
+        // the bound and in the walk
+        const float delta = 0.01f * ext / (float)res + 1e-6f * (1.f + mag);
+        if (!isfinite(delta)) all = true;
+        const float lo3[3] = {blo.x - delta, blo.y - delta, blo.z - delta};
+        const float hi3[3] = {bhi.x + delta, bhi.y + delta, bhi.z + delta};
+        for (int a = 0; a < 3; a++) {
+            const float cell = (hi3[a] - lo3[a]) / (float)res;
+            gp[GP_LO + a] = lo3[a];
+            gp[GP_CELL + a] = cell;
+            gp[GP_ICELL + a] = 1.f / cell;
+            if (!(cell > 0.f) || !isfinite(1.f / cell)) all = true;
+        }
+        gp[GP_RES] = (float)res;
+        gp[GP_ALL] = all ? 1.f : 0.f;
+        gp[GP_M] = m;
+        gp[GP_DELTA] = delta;
+    }
+}
+
+__device__ __forceinline__ int grid_cell_of(float x, float lo, float icell, int res) {
+    const float c = floorf((x - lo) * icell);
+    return (int)fminf(fmaxf(c, 0.f), (float)(res - 1));
+}
+
+// One wavefront per (stroke, piece): sets the stroke's bit in the masks of the cells
+// its grown bound overlaps.
+__global__ __launch_bounds__(256) void grid_insert_kernel(RenderArgs A, const float* __restrict__ gp,
+                                                          unsigned long long* __restrict__ cells) {
+    const int lane = threadIdx.x & 63;
+    const int64_t it = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= (int64_t)A.S * NPIECE || gp[GP_ALL] != 0.f) return;
+    const int k = (int)(it / NPIECE), piece = (int)(it % NPIECE);
+    V3 lo, hi;
+    if (!grid_item_box(A.table + (int64_t)k * REC, piece, gp[GP_M], A.isr, lo, hi)) return;
+    const int res = (int)gp[GP_RES];
+    const float d = gp[GP_DELTA];
+    const int ax = grid_cell_of(lo.x - d, gp[GP_LO], gp[GP_ICELL], res);
+    const int ay = grid_cell_of(lo.y - d, gp[GP_LO + 1], gp[GP_ICELL + 1], res);
+    const int az = grid_cell_of(lo.z - d, gp[GP_LO + 2], gp[GP_ICELL + 2], res);
+    const int nx = grid_cell_of(hi.x + d, gp[GP_LO], gp[GP_ICELL], res) - ax + 1;
+    const int ny = grid_cell_of(hi.y + d, gp[GP_LO + 1], gp[GP_ICELL + 1], res) - ay + 1;
+    const int nz = grid_cell_of(hi.z + d, gp[GP_LO + 2], gp[GP_ICELL + 2], res) - az + 1;
+    const int pos = A.S - 1 - k;
+    const int chunk = pos >> 6;
+    const unsigned long long bit = 1ull << (pos & 63);
+    const int total = nx * ny * nz;
+    for (int q = lane; q < total; q += 64) {
+        const int z = q % nz, y = (q / nz) % ny, x = q / (nz * ny);
+        const int64_t cell = ((int64_t)(ax + x) * res + (ay + y)) * res + (az + z);
+        atomicOr(&cells[cell * A.nch + chunk], bit);
+    }
+}
+
+// One wavefront per (GRID_WALK_RAYS rays, GRID_WALK_CH consecutive chunks).  First the
+// lanes are (ray, piece of its segment): each walks the cells its piece crosses (cell
+// by cell, each crossing depth recomputed from the cell index) and lists them in LDS.
+// Then the lanes are (listed cell, chunk): GRID_WALK_CH lanes read one cell's chunk
+// masks together, and the ray's masks are the OR over its cells.  A ray with more
+// cells than the list holds, or with a non-finite segment, takes every stroke.
+constexpr int GRID_WALK_CH = 8;
+constexpr int GRID_WALK_RAYS = 8;
+constexpr int GRID_WALK_PIECES = 8;
+constexpr int GRID_WALK_LIST = 128;
+constexpr int GRID_WALK_WAVES = 4;
+
+__global__ __launch_bounds__(64 * GRID_WALK_WAVES) void grid_walk_kernel(RenderArgs A, const float* __restrict__ gp,
+                                                                         const float* __restrict__ seg,
+                                                                         const unsigned long long* __restrict__ cells,
+                                                                         unsigned long long* __restrict__ rmask) {
+    static_assert(GRID_WALK_RAYS * GRID_WALK_PIECES == 64 && GRID_WALK_CH * 8 == 64, "lane maps");
+    __shared__ int lists[GRID_WALK_WAVES][GRID_WALK_RAYS][GRID_WALK_LIST];
+    __shared__ int nlist[GRID_WALK_WAVES][GRID_WALK_RAYS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ngrp = (A.nch + GRID_WALK_CH - 1) / GRID_WALK_CH;
+    const int64_t w = (int64_t)blockIdx.x * GRID_WALK_WAVES + wave;
+    const int64_t nblk = (A.R + GRID_WALK_RAYS - 1) / GRID_WALK_RAYS;
+    if (w >= nblk * ngrp) return;  // wave-uniform
+    const int64_t rb = w / ngrp;
+    const int j0 = (int)(w - rb * ngrp) * GRID_WALK_CH;
+    int* cnt = nlist[wave];
+    if (lane < GRID_WALK_RAYS) cnt[lane] = 0;
+    wave_sync();
+    // ---- (1) lanes = (ray r, piece q)
+    {
+        const int r = lane / GRID_WALK_PIECES, q = lane % GRID_WALK_PIECES;
+        const int64_t ray = rb * GRID_WALK_RAYS + r;
+        const bool on = ray < A.R;
+        const int64_t rr = on ? ray : 0;
+        const float tlo = seg[rr * 3], thi = seg[rr * 3 + 1], m = seg[rr * 3 + 2];
+        V3 o = ld3(A.origins + 3 * rr), d = ld3(A.dirs + 3 * rr);
+        if (A.coord_div != 1.f) {
+            o = v3(o.x / A.coord_div, o.y / A.coord_div, o.z / A.coord_div);
+            d = v3(d.x / A.coord_div, d.y / A.coord_div, d.z / A.coord_div);
+        }
+        bool done = !on;
+        if (on && (gp[GP_ALL] != 0.f || !isfinite(tlo) || !isfinite(thi) || !isfinite(m) || !finite3(o) ||
+                   !finite3(d))) {
+            if (q == 0) atomicAdd(&cnt[r], GRID_WALK_LIST + 1);  // every stroke
+            done = true;
+        }
+        const int res = (int)gp[GP_RES];
+        const float lo[3] = {gp[GP_LO], gp[GP_LO + 1], gp[GP_LO + 2]};
+        const float cs[3] = {gp[GP_CELL], gp[GP_CELL + 1], gp[GP_CELL + 2]};
+        const float ic[3] = {gp[GP_ICELL], gp[GP_ICELL + 1], gp[GP_ICELL + 2]};
+        const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+        // crossing depths use the reciprocal direction (within an ulp of the quotient; the
+        // cells are grown by GP_DELTA against that); an axis the ray moves along by less
+        // than 1e-30 per unit depth is never crossed
+        float id[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) id[a] = fabsf(dd[a]) > 1e-30f ? 1.f / dd[a] : 0.f;
+        // the segment, padded, clipped to the grid box (with the reciprocal too: the
+        // strokes' bounds lie GP_DELTA inside the box)
+        const float pad = 1e-4f * (thi - tlo) + 1e-5f * (fabsf(tlo) + fabsf(thi)) + 1e-6f;
+        float t0 = tlo - pad, t1 = thi + pad;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float blo = lo[a], bhi = lo[a] + cs[a] * (float)res;
+            if (id[a] != 0.f) {
+                const float u = (blo - oo[a]) * id[a], v = (bhi - oo[a]) * id[a];
+                t0 = fmaxf(t0, fminf(u, v));
+                t1 = fminf(t1, fmaxf(u, v));
+            } else if (oo[a] < blo || oo[a] > bhi) {
+                t1 = -INFINITY;
+            }
+        }
+        // this lane's piece of [t0, t1], overlapping its neighbours'
+        if (!(t0 <= t1)) done = true;
+        if (!done) {
+            const float len = t1 - t0, ov = 1e-3f * len + 1e-6f * (1.f + fabsf(t0) + fabsf(t1));
+            const float ta = fmaxf(t0, t0 + len * (float)q / GRID_WALK_PIECES - ov);
+            const float tb = fminf(t1, t0 + len * (float)(q + 1) / GRID_WALK_PIECES + ov);
+            t0 = ta;
+            t1 = tb;
+        }
+        int c[3] = {0, 0, 0}, st[3] = {0, 0, 0};
+        float nx[3] = {INFINITY, INFINITY, INFINITY};
+        if (!done) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                c[a] = grid_cell_of(oo[a] + t0 * dd[a], lo[a], ic[a], res);
+                st[a] = id[a] > 0.f ? 1 : (id[a] < 0.f ? -1 : 0);
+                nx[a] = st[a] ? (lo[a] + (float)(c[a] + (st[a] > 0)) * cs[a] - oo[a]) * id[a] : INFINITY;
+            }
+        }
+        const int max_steps = 3 * res + 3;
+        for (int steps = 0; !done; steps++) {
+            const int pos = atomicAdd(&cnt[r], 1);
+            if (pos < GRID_WALK_LIST) lists[wave][r][pos] = (c[0] * res + c[1]) * res + c[2];
+            // step along the axis crossed first (branch-free over the three axes)
+            const bool x0 = nx[0] <= nx[1] && nx[0] <= nx[2];
+            const bool x1 = !x0 && nx[1] <= nx[2];
+            const float tn = x0 ? nx[0] : (x1 ? nx[1] : nx[2]);
+            if (pos >= GRID_WALK_LIST || !(tn <= t1) || steps + 1 >= max_steps) {
+                done = true;
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const bool sel = a == 0 ? x0 : (a == 1 ? x1 : (!x0 && !x1));
+                    if (sel) {
+                        c[a] += st[a];
+                        nx[a] = (lo[a] + (float)(c[a] + (st[a] > 0)) * cs[a] - oo[a]) * id[a];
+                    }
+                }
+                if (c[0] < 0 || c[0] >= res || c[1] < 0 || c[1] >= res || c[2] < 0 || c[2] >= res) done = true;
+            }
+        }
+    }
+    wave_sync();
+    // ---- (2) lanes = (listed cell u, chunk cc), one ray at a time
+    const int u = lane / GRID_WALK_CH, cc = lane % GRID_WALK_CH;
+    const bool has = j0 + cc < A.nch;
+    for (int r = 0; r < GRID_WALK_RAYS; r++) {
+        const int64_t ray = rb * GRID_WALK_RAYS + r;
+        if (ray >= A.R) break;  // wave-uniform
+        const int n = cnt[r];
+        unsigned long long acc = 0ull;
+        if (n > GRID_WALK_LIST) {
+            acc = ~0ull;
+        } else {
+            for (int k0 = 0; k0 < n; k0 += 8) {
+                const int k = k0 + u;
+                if (k < n && has) acc |= cells[(int64_t)lists[wave][r][k] * A.nch + j0 + cc];
+            }
+        }
+#pragma unroll
+        for (int off = GRID_WALK_CH; off < 64; off <<= 1) {
+            const unsigned l32 = __shfl_xor((unsigned)acc, off, 64), h32 = __shfl_xor((unsigned)(acc >> 32), off, 64);
+            acc |= ((unsigned long long)h32 << 32) | l32;
+        }
+        if (u == 0 && has) {
+            const int nvalid = min(64, A.S - 64 * (j0 + cc));
+            rmask[ray * A.nch + j0 + cc] = acc & (nvalid >= 64 ? ~0ull : ((1ull << nvalid) - 1ull));
+        }
+    }
+}
+
+// per wave: depths [64 NS]; per batch candidate: interval lo / hi (order-preserving
+// ints), state (hit | curve << 1 | passing pieces << 16) and (chunk << 6 | lane); curve
+// -> candidate map [64]; capsule candidates [128]; per task chunk: first candidate,
+// hits and pairs written
+__host__ __device__ constexpr int cullg_wave_floats(int ns) {
+    return 64 * ns + GRID_CAP * 4 + 64 + 128 + 3 * (GRID_STRIDE + 1);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64 * CWAVES) void render_cull_grid_kernel(RenderArgs A,
+                                                                       const unsigned long long* __restrict__ rmask,
+                                                                       int* __restrict__ counts,
+                                                                       int* __restrict__ slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ray = (int64_t)blockIdx.x * CWAVES + wave;
+    if (ray >= A.R) return;
+    WaveMem w;
+    w.ts = smem + wave * cullg_wave_floats(NS);
+    int* ilo = (int*)(w.ts + 64 * NS);  // [GRID_CAP]
+    int* ihi = ilo + GRID_CAP;
+    int* ist = ihi + GRID_CAP;
+    int* cl = ist + GRID_CAP;           // candidate -> chunk << 6 | lane
+    int* src = cl + GRID_CAP;           // [64]
+    int* cand = src + 64;               // [128]
+    int* cfirst = cand + 128;           // [GRID_STRIDE + 1] first candidate of each chunk
+    int* chits = cfirst + GRID_STRIDE + 1;
+    int* cpairs = chits + GRID_STRIDE + 1;
+    RayConst rc = load_ray(A, ray);
+    float ts[NS];
+    bool valid[NS];
+    const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
+    if (!sorted) unsorted_range<NS>(A, rc, ts, valid);
+    const int nch = A.nch;
+    // lane j: the mask of chunk j and the candidates before it
+    const unsigned long long mj = lane < nch ? rmask[ray * nch + lane] : 0ull;
+    const int pc = __popcll(mj);
+    const int pincl = wave_incl_scan(pc, lane);
+    if (lane < nch) {
+        cfirst[lane] = pincl - pc;
+        chits[lane] = 0;
+        cpairs[lane] = 0;
+    }
+    const int total = __shfl(pincl, 63, 64);
+    if (lane == 0) cfirst[nch] = total;
+    wave_sync();
+    const int64_t slot0 = ray * A.nch_stride;
+    for (int jb = 0, cb = 0; jb < nch; ) {
+        // the batch: chunks [jb, je) with at most GRID_CAP candidates (at least one chunk)
+        const unsigned long long fit = __ballot(lane >= jb && lane < nch && pincl - cb <= GRID_CAP);
+        int je = fit ? 64 - __builtin_clzll(fit) : jb + 1;
+        if (je <= jb) je = jb + 1;
+        const int nc = cfirst[je] - cb;
+        for (int j = jb; j < je; j++) {
+            const unsigned long long mm = __shfl(mj, j, 64);  // 64-bit shuffle of lane j's mask
+            if ((mm >> lane) & 1ull) cl[cfirst[j] - cb + __popcll(mm & ((1ull << lane) - 1ull))] = (j << 6) | lane;
+        }
+        wave_sync();
+        // (1) per candidate: cull the stroke; for hull-hit curves test each piece's chord
+        //     box, one lane per piece, and keep the mask of passing pieces
+        for (int r0 = 0; r0 < nc; r0 += 64) {
+            const int q = r0 + lane;
+            const bool on = q < nc;
+            const int code = on ? cl[q] : 0;
+            const int k = A.S - 1 - ((code >> 6) * 64 + (code & 63));
+            float ta = 0.f, tb = -1.f;
+            bool curve = false;
+            const bool hit = on && cull_stroke(A.table + (int64_t)k * REC, rc.oc, rc.dc, rc.di, rc.m, A.isr,
+                                               rc.tlo, rc.thi, ta, tb, curve);
+            const unsigned long long cm = __ballot(hit && curve);
+            int pmask = 0;
+            if (cm) {
+                const int crank = __popcll(cm & ((1ull << lane) - 1ull));
+                if (hit && curve) src[crank] = k;
+                wave_sync();
+                const int ncv = __popcll(cm);
+                for (int q0 = 0; q0 < ncv; q0 += 64 / NPIECE) {
+                    const int cs = q0 + lane / NPIECE;
+                    const int piece = lane % NPIECE;
+                    bool bh = false;
+                    if (cs < ncv) {
+                        const float* rec = A.table + (int64_t)src[cs] * REC;
+                        const float e = rec[R_RMAX] + rc.m + rec[R_DEV + piece];
+                        const V3 a = ld3(rec + R_PTS + 3 * piece), b = ld3(rec + R_PTS + 3 * piece + 3);
+                        float p0 = rc.tlo, p1 = rc.thi;
+                        slab_inv(rc.oc.x, rc.di.x, fminf(a.x, b.x) - e, fmaxf(a.x, b.x) + e, p0, p1);
+                        slab_inv(rc.oc.y, rc.di.y, fminf(a.y, b.y) - e, fmaxf(a.y, b.y) + e, p0, p1);
+                        slab_inv(rc.oc.z, rc.di.z, fminf(a.z, b.z) - e, fmaxf(a.z, b.z) + e, p0, p1);
+                        bh = p0 <= p1;
+                    }
+                    const unsigned long long bm = __ballot(bh);
+                    if (cs < ncv && piece == 0) cand[cs] = (int)((bm >> (lane & ~(NPIECE - 1))) & 0xffffull);
+                }
+                wave_sync();
+                if (hit && curve) pmask = cand[crank];
+                wave_sync();
+            }
+            if (on) {
+                const bool live = hit && (!curve || pmask != 0);
+                ilo[q] = curve ? ord_int(1e30f) : ord_int(ta);
+                ihi[q] = curve ? ord_int(-1e30f) : ord_int(tb);
+                ist[q] = (live ? 1 : 0) | (curve ? 2 : 0) | (pmask << 16);
+            }
+        }
+        wave_sync();
+        // (2) the passing pieces against their chord capsules, 64 at a time; each hit
+        //     capsule widens its curve's interval (LDS min / max)
+        int np_c = 0;  // pending capsule candidates, < 64 between rounds
+        const int nrounds = (nc + 63) / 64;
+        for (int rr = 0; rr <= nrounds; rr++) {  // the last pass only flushes
+            const int r0 = rr * 64;
+            unsigned pm = 0u;
+            int npc = 0, incl = 0, tot = 0;
+            const bool more = rr < nrounds;
+            if (more) {
+                const int q = r0 + lane;
+                const int st = q < nc ? ist[q] : 0;
+                pm = (st & 1) ? ((unsigned)st >> 16) : 0u;
+                npc = __popc(pm);
+                incl = wave_incl_scan(npc, lane);
+                tot = __shfl(incl, 63, 64);
+            }
+            int done = 0;
+            do {
+                const int take = min(tot - done, 128 - np_c);
+                int idx = incl - npc;
+                for (unsigned mm = pm; mm; mm &= mm - 1, idx++)
+                    if (idx >= done && idx < done + take)
+                        cand[np_c + idx - done] = ((r0 + lane) << 4) | __builtin_ctz(mm);
+                np_c += take;
+                done += take;
+                wave_sync();
+                while (np_c >= 64 || (!more && np_c > 0)) {
+                    const int nr = min(np_c, 64);
+                    if (lane < nr) {
+                        const int cd = cand[lane];
+                        const int cq = cd >> 4, pcs = cd & 15;
+                        const int code = cl[cq];
+                        const float* rec = A.table + (int64_t)(A.S - 1 - ((code >> 6) * 64 + (code & 63))) * REC;
+                        float p0, p1;
+                        if (capsule_interval(rc.oc, rc.dc, rc.idd, ld3(rec + R_PTS + 3 * pcs),
+                                             ld3(rec + R_PTS + 3 * pcs + 3), rec[R_RMAX] + rc.m + rec[R_DEV + pcs],
+                                             p0, p1)) {
+                            p0 = fmaxf(p0, rc.tlo);
+                            p1 = fminf(p1, rc.thi);
+                            if (p0 <= p1) {
+                                atomicMin(&ilo[cq], ord_int(p0));
+                                atomicMax(&ihi[cq], ord_int(p1));
+                            }
+                        }
+                    }
+                    const int rest = np_c - nr;
+                    const int mv = lane < rest ? cand[64 + lane] : 0;
+                    wave_sync();
+                    if (lane < rest) cand[lane] = mv;
+                    np_c = rest;
+                    wave_sync();
+                }
+            } while (done < tot);
+        }
+        wave_sync();
+        // (3) per candidate: the sample range; hits are written per chunk in lane order
+        for (int r0 = 0; r0 < nc; r0 += 64) {
+            const int q = r0 + lane;
+            const bool on = q < nc;
+            const int code = on ? cl[q] : 0;
+            const int j = code >> 6, l = code & 63;
+            const int k = A.S - 1 - (j * 64 + l);
+            const int st = on ? ist[q] : 0;
+            bool hit = (st & 1) != 0;
+            int i0 = 0, cnt = 0, cls = 2;
+            if (hit) {
+                float ta = ord_float(ilo[q]), tb = ord_float(ihi[q]);
+                if (st & 2) {
+                    hit = ta <= tb;
+                    if (hit) pad_interval(ta, tb, ta, tb);
+                }
+                if (hit) {
+                    if (sorted) {
+                        i0 = count_below<NS, false>(w.ts, ta);
+                        cnt = count_below<NS, true>(w.ts, tb) - i0;
+                    } else {
+                        cnt = A.n;
+                    }
+                    hit = cnt > 0;
+                    cls = stroke_class(A.table + (int64_t)k * REC, (st & 2) != 0);
+                }
+            }
+            const unsigned long long hm = __ballot(hit);
+            if (hit) {
+                // candidates of one chunk are contiguous: rank among this round's hits of the
+                // chunk, after the chunk's hits of earlier rounds
+                const int s0 = max(cfirst[j] - cb - r0, 0);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                const unsigned long long seg0 = s0 > 0 ? (1ull << s0) - 1ull : 0ull;
+                const int rank = chits[j] + __popcll(hm & below & ~seg0);
+                slots[(slot0 + j) * 64 + rank] = pack_hit(l, i0, cnt, cls);
+            }
+            wave_sync();
+            if (hit) {
+                atomicAdd(&chits[j], 1);
+                atomicAdd(&cpairs[j], cnt);
+            }
+            wave_sync();
+        }
+        cb += nc;
+        jb = je;
+    }
+    if (lane < nch) counts[slot0 + lane] = chits[lane] | (cpairs[lane] << 8);
+}
+
+// ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
 // PRE: the hits come from the culling pass (render_cull_kernel); otherwise the
@@ -2335,14 +2949,15 @@ static int g_max_lds = 0;
 // that trigger an evaluation window, backward persistent blocks per resident block,
 // the batch backward's blocks per resident block, and chunks per culling task.
 // TUNE_BWD_QUEUE: the adjoint queues live pairs per SDF family across rays (1) or
-// evaluates each window's pairs as they come (0)
+// evaluates each window's pairs as they come (0).  TUNE_CULL_GRID: the culling pass of
+// one scene tests only the strokes of its ray's grid cells (1) or every stroke (0).
 enum {
     TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_CULL_GROUP = 3, TUNE_BWD_QUEUE = 4,
-    TUNE_COUNT = 5
+    TUNE_CULL_GRID = 5, TUNE_COUNT = 6
 };
-static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1, 1};
-static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1, 0};
-static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16, 1};
+static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1, 1, 1};
+static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1, 0, 0};
+static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16, 1, 1};
 
 typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
 typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*,
@@ -2552,6 +3167,101 @@ static int launch_cull(RenderArgs& A, int ns, void* workspace, int64_t workspace
     return N3DS_OK;
 }
 
+// Grid of the grid-assisted culling pass, behind the culling workspace (256-byte
+// aligned pieces): parameter block, per-block margins, per-ray segments, cell masks
+// [res^3, nch] and per-ray masks [R, nch].
+struct GridWs {
+    float* gp;
+    float* part;
+    float* seg;
+    unsigned long long* cells;
+    unsigned long long* rmask;
+    int res;
+    int64_t nparts, ncell_words, bytes;
+};
+
+static int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+// byte offsets of the pieces (parameters, margins, segments, cells, ray masks, end)
+static void grid_offsets(int64_t n_rays, int nch, int64_t* off) {
+    const int res = grid_res(nch);
+    off[0] = 0;
+    off[1] = off[0] + align256(GP_COUNT * 4);
+    off[2] = off[1] + align256((n_rays + GRID_RAY_WAVES * GRID_RAYS_PER_WAVE - 1) / (GRID_RAY_WAVES * GRID_RAYS_PER_WAVE) * 4);
+    off[3] = off[2] + align256(n_rays * 12);
+    off[4] = off[3] + align256((int64_t)res * res * res * nch * 8);
+    off[5] = off[4] + align256(n_rays * nch * 8);
+}
+
+static GridWs grid_layout(char* base, int64_t n_rays, int nch) {
+    GridWs g;
+    int64_t off[6];
+    grid_offsets(n_rays, nch, off);
+    g.res = grid_res(nch);
+    g.nparts = (n_rays + GRID_RAY_WAVES * GRID_RAYS_PER_WAVE - 1) / (GRID_RAY_WAVES * GRID_RAYS_PER_WAVE);
+    g.ncell_words = (int64_t)g.res * g.res * g.res * nch;
+    g.gp = (float*)(base + off[0]);
+    g.part = (float*)(base + off[1]);
+    g.seg = (float*)(base + off[2]);
+    g.cells = (unsigned long long*)(base + off[3]);
+    g.rmask = (unsigned long long*)(base + off[4]);
+    g.bytes = off[5];
+    return g;
+}
+
+static int64_t grid_workspace_bytes(int64_t n_rays, int nch) {
+    if (nch < 1 || nch > GRID_MAX_NCH) return 0;
+    int64_t off[6];
+    grid_offsets(n_rays, nch, off);
+    return off[5];
+}
+
+typedef void (*CullGridKernel)(RenderArgs, const unsigned long long*, int*, int*);
+typedef void (*GridRaysKernel)(RenderArgs, float*, float*, unsigned long long*, int64_t);
+
+// The culling pass of one scene with the grid: rays, setup, insertion, walk, then the
+// candidate culling.  Launches nothing (and leaves *used false) when the grid does not
+// apply: a batch, no strokes, more than GRID_MAX_NCH chunks, a workspace without room.
+static int launch_cull_grid(RenderArgs& A, int ns, void* workspace, int64_t workspace_bytes, hipStream_t s,
+                            bool* used) {
+    *used = false;
+    const int64_t cb = align256(cull_workspace_bytes(A.R, A.nch_stride));
+    if (!workspace || !g_tune[TUNE_CULL_GRID] || A.scenes || A.nch < 1 || A.nch > GRID_MAX_NCH ||
+        A.nch_stride != A.nch || workspace_bytes < cb + grid_workspace_bytes(A.R, A.nch))
+        return N3DS_OK;
+    N3DS_REQUIRE(A.R / CWAVES < (1ll << 31) && A.R * (int64_t)(A.nch + 7) / 8 * 64 / 256 < (1ll << 31) &&
+                 (int64_t)A.S * NPIECE / 4 < (1ll << 31), "too many rays or strokes for the culling grid");
+    int* counts = (int*)workspace;
+    int* slots = counts + A.R * A.nch_stride;
+    const GridWs g = grid_layout((char*)workspace + cb, A.R, A.nch);
+    static const GridRaysKernel rk[4] = {grid_rays_kernel<1>, grid_rays_kernel<2>, grid_rays_kernel<3>,
+                                         grid_rays_kernel<4>};
+    static const CullGridKernel ck[4] = {render_cull_grid_kernel<1>, render_cull_grid_kernel<2>,
+                                         render_cull_grid_kernel<3>, render_cull_grid_kernel<4>};
+    hipLaunchKernelGGL(rk[ns - 1], dim3((unsigned)g.nparts), dim3(64 * GRID_RAY_WAVES), 0, s, A, g.seg, g.part,
+                       g.cells, g.ncell_words);
+    N3DS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(grid_setup_kernel, dim3(1), dim3(GRID_SETUP_THREADS), 0, s, A, (const float*)g.part,
+                       (int)g.nparts, g.res, g.gp);
+    N3DS_LAUNCH_CHECK();
+    const int64_t items = (int64_t)A.S * NPIECE;
+    hipLaunchKernelGGL(grid_insert_kernel, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, A,
+                       (const float*)g.gp, g.cells);
+    N3DS_LAUNCH_CHECK();
+    const int64_t walks = (A.R + GRID_WALK_RAYS - 1) / GRID_WALK_RAYS * ((A.nch + GRID_WALK_CH - 1) / GRID_WALK_CH);
+    hipLaunchKernelGGL(grid_walk_kernel, dim3((unsigned)((walks + GRID_WALK_WAVES - 1) / GRID_WALK_WAVES)),
+                       dim3(64 * GRID_WALK_WAVES), 0, s, A,
+                       (const float*)g.gp, (const float*)g.seg, (const unsigned long long*)g.cells, g.rmask);
+    N3DS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(ck[ns - 1], dim3((unsigned)((A.R + CWAVES - 1) / CWAVES)), dim3(64 * CWAVES),
+                       CWAVES * cullg_wave_floats(ns) * 4, s, A, (const unsigned long long*)g.rmask, counts, slots);
+    N3DS_LAUNCH_CHECK();
+    A.ccounts = counts;
+    A.cslots = slots;
+    *used = true;
+    return N3DS_OK;
+}
+
 N3DS_EXPORT int n3ds_render_extras(const float* state, const float* tdist, const float* directions,
                                    const float* t_far, int64_t n_rays, int n_samples, int opaque_background,
                                    float* extras_out, void* stream) {
@@ -2567,7 +3277,8 @@ N3DS_EXPORT int n3ds_render_extras(const float* state, const float* tdist, const
 
 N3DS_EXPORT int64_t n3ds_render_forward_workspace_bytes(int64_t n_rays, int max_strokes) {
     if (n_rays < 0 || max_strokes < 0) return -1;
-    return cull_workspace_bytes(n_rays, (max_strokes + 63) / 64);
+    const int nch = (max_strokes + 63) / 64;
+    return align256(cull_workspace_bytes(n_rays, nch)) + grid_workspace_bytes(n_rays, nch);
 }
 
 N3DS_EXPORT int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* shape_params,
@@ -2600,9 +3311,14 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     if (st) return st;
     dim3 grid((unsigned)((n_rays + FWAVES - 1) / FWAVES)), block(64 * FWAVES);
     const int ns = (n_samples + 63) / 64;
-    st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream,
-                     n_rays * (int64_t)((A.nch_stride + A.cull_group - 1) / A.cull_group));
+    bool gridded = false;
+    st = launch_cull_grid(A, ns, workspace, workspace_bytes, (hipStream_t)stream, &gridded);
     if (st) return st;
+    if (!gridded) {
+        st = launch_cull(A, ns, workspace, workspace_bytes, (hipStream_t)stream,
+                         n_rays * (int64_t)((A.nch_stride + A.cull_group - 1) / A.cull_group));
+        if (st) return st;
+    }
     hipLaunchKernelGGL(A.ccounts ? fwd_pre_kernel_ptr(ns, color_id) : fwd_kernel_ptr(ns, color_id), grid, block,
                        fwd_lds_bytes(ns, color_id), (hipStream_t)stream,
                        A, rgb_out, acc_out, depth_out, state_out, counters);

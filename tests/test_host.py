@@ -159,6 +159,7 @@ def test_render_tuning_parameters():
     assert lib.n3ds_render_tuning(0, 5000) == -1 and b'outside' in lib.n3ds_last_error()
     assert lib.n3ds_render_tuning(7, 1) == -1
     assert old['bwd_queue'] == 1 and lib.n3ds_render_tuning(4, 2) == -1
+    assert old['cull_grid'] == 1 and lib.n3ds_render_tuning(5, 2) == -1
     assert autotune.get('flush_pairs') == 192
     for k, v in old.items():
         autotune.set(k, v)

@@ -173,28 +173,40 @@ __device__ float point_segment_dist(V3 p, V3 a, V3 b) {
     return safe_len(ap - ab * h);
 }
 
-__global__ void render_prep_kernel(float* __restrict__ table, const int* __restrict__ ids,
-                                   const float* __restrict__ sp, int S, int Pmax, int* __restrict__ status) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= S) return;
-    float* rec = table + (int64_t)k * REC;
-    for (int i = 0; i < REC; i++) rec[i] = 0.f;
+// One stroke per NPIECE consecutive lanes (a curve's pieces one per lane).
+__global__ __launch_bounds__(128) void render_prep_kernel(float* __restrict__ table, const int* __restrict__ ids,
+                                                          const float* __restrict__ sp, int S, int Pmax,
+                                                          int* __restrict__ status) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = gt / NPIECE;
+    const int pl = (int)(gt % NPIECE);
+    const bool on = k < S;  // the same for the NPIECE lanes of a stroke
+    float* rec = table + (on ? k : 0) * REC;
+    if (on)
+        for (int i = pl; i < REC; i += NPIECE) rec[i] = 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!on) return;
     uint32_t id = (uint32_t)ids[k];
     int base = (int)(id >> 4);
     uint32_t flags = id & 15u;
     int P = num_shape_params(id);
     if (base >= NUM_BASE || P > Pmax || ((flags & F_SS) && (flags & F_MS)) || (is_curve(base) && flags)) {
-        atomicOr(status, 1);
+        if (pl == 0) atomicOr(status, 1);
         base = SPHERE; flags = 0; P = 0;
     }
-    const float* row = sp + (int64_t)k * Pmax;
+    const float* row = sp + k * Pmax;
     int nb = num_base_params(base);
-    rec[R_BASE] = fbits(base);
-    rec[R_FLAGS] = fbits((int)flags);
-    rec[R_P] = fbits(P);
-    rec[R_NB] = fbits(nb);
-    rec[R_CURVE] = fbits(is_curve(base) ? 1 : 0);
+    if (pl == 0) {
+        rec[R_BASE] = fbits(base);
+        rec[R_FLAGS] = fbits((int)flags);
+        rec[R_P] = fbits(P);
+        rec[R_NB] = fbits(nb);
+        rec[R_CURVE] = fbits(is_curve(base) ? 1 : 0);
+    }
     if (!is_curve(base)) {
+        if (pl != 0) return;
         Frame F = load_frame(row, P, flags);
         rec[R_OFFS] = fbits(F.off_scale);
         rec[R_OFFA] = fbits(F.off_angle);
@@ -228,8 +240,10 @@ __global__ void render_prep_kernel(float* __restrict__ table, const int* __restr
         rec[R_E0 + 0] = e0.x * 1.0001f + 1e-5f; rec[R_E0 + 1] = e0.y * 1.0001f + 1e-5f; rec[R_E0 + 2] = e0.z * 1.0001f + 1e-5f;
         rec[R_E1 + 0] = e1.x * 1.0001f; rec[R_E1 + 1] = e1.y * 1.0001f; rec[R_E1 + 2] = e1.z * 1.0001f;
     } else {
-        rec[R_OFFS] = fbits(-1); rec[R_OFFA] = fbits(-1); rec[R_OFFT] = fbits(-1);
-        for (int i = 0; i < nb; i++) rec[R_CP + i] = row[i];
+        if (pl == 0) {
+            rec[R_OFFS] = fbits(-1); rec[R_OFFA] = fbits(-1); rec[R_OFFT] = fbits(-1);
+            for (int i = 0; i < nb; i++) rec[R_CP + i] = row[i];
+        }
         V3 K[4];
         float r1, r2;
         if (base == QUAD_BEZIER) {
@@ -246,33 +260,43 @@ __global__ void render_prep_kernel(float* __restrict__ table, const int* __restr
         // pieces as chord capsules: the piece [j, j+1] / NPIECE lies within dev_j of the chord
         // between its end points (the distance to a segment is convex, so its maximum over the
         // hull of the piece's control points is at a control point); the hull of all control
-        // points bounds the whole curve
+        // points bounds the whole curve.  Lane j takes piece j.
+        const int j = pl;
+        V3 c[4];
+        curve_piece_ctrl(base, K, (float)j / NPIECE, (float)(j + 1) / NPIECE, c);
         float hull[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
-        for (int j = 0; j < NPIECE; j++) {
-            V3 c[4];
-            curve_piece_ctrl(base, K, (float)j / NPIECE, (float)(j + 1) / NPIECE, c);
-            float mag = 0.f;
-            for (int q = 0; q < 4; q++) {
-                hull[0] = fminf(hull[0], c[q].x); hull[1] = fminf(hull[1], c[q].y); hull[2] = fminf(hull[2], c[q].z);
-                hull[3] = fmaxf(hull[3], c[q].x); hull[4] = fmaxf(hull[4], c[q].y); hull[5] = fmaxf(hull[5], c[q].z);
-                mag = fmaxf(mag, fmaxf(fabsf(c[q].x), fmaxf(fabsf(c[q].y), fabsf(c[q].z))));
-            }
-            const float dev = fmaxf(point_segment_dist(c[1], c[0], c[3]), point_segment_dist(c[2], c[0], c[3]));
-            rec[R_DEV + j] = dev * 1.001f + 1e-5f * (1.f + mag);
-            rec[R_PTS + 3 * j + 0] = c[0].x; rec[R_PTS + 3 * j + 1] = c[0].y; rec[R_PTS + 3 * j + 2] = c[0].z;
-            if (j == NPIECE - 1) {
-                rec[R_PTS + 3 * NPIECE + 0] = c[3].x;
-                rec[R_PTS + 3 * NPIECE + 1] = c[3].y;
-                rec[R_PTS + 3 * NPIECE + 2] = c[3].z;
+        float mag = 0.f;
+        for (int q = 0; q < 4; q++) {
+            hull[0] = fminf(hull[0], c[q].x); hull[1] = fminf(hull[1], c[q].y); hull[2] = fminf(hull[2], c[q].z);
+            hull[3] = fmaxf(hull[3], c[q].x); hull[4] = fmaxf(hull[4], c[q].y); hull[5] = fmaxf(hull[5], c[q].z);
+            mag = fmaxf(mag, fmaxf(fabsf(c[q].x), fmaxf(fabsf(c[q].y), fabsf(c[q].z))));
+        }
+        const float dev = fmaxf(point_segment_dist(c[1], c[0], c[3]), point_segment_dist(c[2], c[0], c[3]));
+        rec[R_DEV + j] = dev * 1.001f + 1e-5f * (1.f + mag);
+        rec[R_PTS + 3 * j + 0] = c[0].x; rec[R_PTS + 3 * j + 1] = c[0].y; rec[R_PTS + 3 * j + 2] = c[0].z;
+        if (j == NPIECE - 1) {
+            rec[R_PTS + 3 * NPIECE + 0] = c[3].x;
+            rec[R_PTS + 3 * NPIECE + 1] = c[3].y;
+            rec[R_PTS + 3 * NPIECE + 2] = c[3].z;
+        }
+        // min / max over the stroke's NPIECE lanes (aligned groups, so xor stays inside)
+#pragma unroll
+        for (int off = 1; off < NPIECE; off <<= 1) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                hull[a] = fminf(hull[a], __shfl_xor(hull[a], off, 64));
+                hull[3 + a] = fmaxf(hull[3 + a], __shfl_xor(hull[3 + a], off, 64));
             }
         }
-        for (int a = 0; a < 3; a++) {
-            const float pad = 1e-5f * (1.f + fmaxf(fabsf(hull[a]), fabsf(hull[3 + a])));
-            hull[a] -= pad;
-            hull[3 + a] += pad;
+        if (pl == 0) {
+            for (int a = 0; a < 3; a++) {
+                const float pad = 1e-5f * (1.f + fmaxf(fabsf(hull[a]), fabsf(hull[3 + a])));
+                hull[a] -= pad;
+                hull[3 + a] += pad;
+            }
+            for (int a = 0; a < 6; a++) rec[R_HULL + a] = hull[a];
+            rec[R_RMAX] = fmaxf(fmaxf(r1, r2), 0.f) * 1.0001f + 1e-6f;
         }
-        for (int a = 0; a < 6; a++) rec[R_HULL + a] = hull[a];
-        rec[R_RMAX] = fmaxf(fmaxf(r1, r2), 0.f) * 1.0001f + 1e-6f;
     }
 }
 
@@ -2885,7 +2909,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
 // Sum per-block partial rows into the parameter-gradient tensors (+=).  blockIdx.y
 // takes a slice of RED_PARTS consecutive rows; slices meet in float atomics (one
 // slice: a plain read-modify-write).
-constexpr int RED_PARTS = 64;
+constexpr int RED_PARTS = 16;
 
 __device__ __forceinline__ void add_grad(float* p, float v, bool atomic) {
     if (atomic) atomicAdd(p, v);
@@ -2900,6 +2924,7 @@ __global__ void render_reduce_kernel(const float* __restrict__ partials, int npa
     if (i >= (int64_t)S * PG) return;
     const int b0 = blockIdx.y * RED_PARTS, b1 = min(nparts, b0 + RED_PARTS);
     float s = 0.f;
+#pragma unroll 8
     for (int b = b0; b < b1; b++) s += partials[(int64_t)b * S * PG + i];
     const bool atomic = gridDim.y > 1;
     int k = (int)(i / PG), c = (int)(i - (int64_t)k * PG);
@@ -3285,7 +3310,9 @@ N3DS_EXPORT int n3ds_render_prepare(float* table, const int32_t* stroke_ids, con
                                     int n_strokes, int shape_width, int* status, void* stream) {
     N3DS_REQUIRE(n_strokes >= 0 && shape_width >= 1 && shape_width <= 23, "bad stroke table sizes");
     if (n_strokes == 0) return N3DS_OK;
-    hipLaunchKernelGGL(render_prep_kernel, dim3((n_strokes + 127) / 128), dim3(128), 0, (hipStream_t)stream, table,
+    static_assert(64 % NPIECE == 0 && 128 % NPIECE == 0, "a stroke's lanes share a wavefront");
+    hipLaunchKernelGGL(render_prep_kernel, dim3((unsigned)(((int64_t)n_strokes * NPIECE + 127) / 128)), dim3(128), 0,
+                       (hipStream_t)stream, table,
                        stroke_ids, shape_params, n_strokes, shape_width, status);
     N3DS_LAUNCH_CHECK();
     return N3DS_OK;

@@ -257,6 +257,15 @@ int64_t n3ds_render_forward_workspace_bytes(int64_t n_rays, int max_strokes);
  * device; -1 on a HIP error). */
 int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_strokes, int shape_width, int color_width);
 
+/* Floats of device workspace the split adjoint needs, or 0 when it does not apply
+ * (no counterpart in the reference: the reference's adjoint is the per-operator chain).
+ * With the forward's kept hits (hit_dir != NULL), n3ds_render_backward runs the
+ * adjoint as a per-ray kernel that lists the live (sample, stroke) pairs' adjoint
+ * records per SDF family and a kernel over those records, when its workspace holds
+ * max(this, n3ds_render_backward_workspace_floats(...)) floats; same results up to
+ * the float summation order of the gradients. */
+int64_t n3ds_render_backward_split_floats(int64_t n_rays, int n_strokes, int shape_width, int color_width);
+
 /* Backward: accumulates dL/d(shape_params) [S,shape_width], dL/d(color_params)
  * [S,color_width] and dL/d(density_params) [S] for upstream grad_rgb [R,3],
  * grad_acc [R] (or NULL) and grad_depth [R] (or NULL), from the forward's

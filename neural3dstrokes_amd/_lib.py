@@ -47,6 +47,7 @@ SIGNATURES = {
     'n3ds_render_forward': (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _F, _I32, _I32,
                                    _I32, _F, _P, _P, _I32, _F, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P]),
     'n3ds_render_forward_workspace_bytes': (_I64, [_I64, _I32]),
+    'n3ds_render_backward_split_floats': (_I64, [_I64, _I32, _I32, _I32]),
     'n3ds_render_extras': (_I32, [_P, _P, _P, _P, _I64, _I32, _I32, _P, _P]),
     'n3ds_render_tuning': (_I32, [_I32, _I32]),
     'n3ds_render_backward_workspace_floats': (_I64, [_I64, _I32, _I32, _I32]),

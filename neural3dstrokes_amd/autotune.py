@@ -14,7 +14,12 @@ Tunes, by timing the real workload on the device, the parameters exposed by
   each window's pairs as they come (0);
 * ``cull_grid``: the culling pass of one scene tests each ray against the
   strokes binned in the grid cells its segment crosses (1), or against every
-  stroke (0); the hits are the same.
+  stroke (0); the hits are the same;
+* ``bwd_split``: with the forward's kept hits, the adjoint runs as a per-ray
+  kernel that lists its live pairs' adjoint records per SDF family and a kernel
+  over those records (1), or as one kernel (0);
+* ``split_cap``: records per ray and family the split adjoint's lists hold (a
+  fuller list makes the one-kernel adjoint take every ray).
 
 Results do not depend on them beyond float summation order.  ``tune`` runs a
 caller-supplied step over a small grid (coordinate descent, one parameter at a
@@ -25,9 +30,11 @@ import torch
 
 from . import _lib
 
-PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3, 'bwd_queue': 4, 'cull_grid': 5}
+PARAMS = {'flush_pairs': 0, 'bwd_oversub': 1, 'batch_oversub': 2, 'cull_group': 3, 'bwd_queue': 4, 'cull_grid': 5,
+          'bwd_split': 6, 'split_cap': 7}
 SPACE = {'flush_pairs': (32, 48, 64, 96, 128, 192, 256), 'bwd_oversub': (1, 2, 4), 'batch_oversub': (2, 4, 8, 16),
-         'cull_group': (1, 2, 4, 8, 12, 16), 'bwd_queue': (0, 1), 'cull_grid': (0, 1)}
+         'cull_group': (1, 2, 4, 8, 12, 16), 'bwd_queue': (0, 1), 'cull_grid': (0, 1), 'bwd_split': (0, 1),
+         'split_cap': (16, 32, 64)}
 _cache = {}
 
 

@@ -111,6 +111,18 @@ struct RenderArgs {
     int64_t ntask;    // culling tasks: rays x chunk groups (a batch: each scene's own chunks)
     int cull_group;   // chunks culled by one culling task
     int flush_pairs;  // pending pairs that trigger evaluation
+    // split adjoint (render_bwd_kernel MODE 2, then render_bwd_pairs_kernel): per SDF
+    // family and MODE 2 wavefront a field-major region of sp_cap adjoint records
+    // (stroke, ray, depth, dL/dalpha, dL/d(density), dL/d(colour) x 3), sp_count [NCLASS,
+    // sp_nw] the 64-record groups each region holds, sp_ovf set when a region overflowed
+    // (the fused kernel then takes every ray), fbflag [R] the rays left to the fused
+    // kernel; nullptr outside the split adjoint
+    float* sp_list;
+    int* sp_count;
+    int* sp_ovf;
+    int* fbflag;
+    int64_t sp_cap;
+    int sp_nw;
 };
 
 constexpr int SCENE_REC = 10;
@@ -2385,8 +2397,67 @@ __device__ __forceinline__ void queued_group(const RenderArgs& A, const WaveMem&
     reduce_runs<NG>(A, w, lane, nl, k, G, dst, PG);
 }
 
-// QUEUE: the pending-pair queues (queues != nullptr); without, gradient groups run window by window
-template <int NS, int CID, bool QUEUE>
+__host__ __device__ constexpr int pairs_wave_floats(int ng) { return 64 * (ng + 1) + 64; }
+constexpr int PAIRS_WAVES = 12;  // wavefronts per block of the pair kernel (one block per CU)
+
+// Split adjoint, second kernel: the records of render_bwd_kernel<MODE 2>, 64 of one SDF
+// family per wavefront, through pair_grad; each stroke's run of lanes is summed into the
+// block's rows (LDS, or the global rows) as in the fused kernel.  Does nothing when a
+// region overflowed (the fused kernel then takes every ray).
+template <int CID>
+__global__ __launch_bounds__(64 * PAIRS_WAVES) void render_bwd_pairs_kernel(RenderArgs A, float* __restrict__ partials,
+                                                                       int use_lds, float* __restrict__ gdirect) {
+    extern __shared__ float smem[];  // [S * PG] accumulators (use_lds) + per-wavefront regions
+    constexpr int NG = 15 + num_color_params(CID);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = blockDim.x >> 6;
+    const int PG = A.Pmax + A.Q + 1;
+    const int64_t nacc = use_lds ? (int64_t)A.S * PG : 0;
+    float* dst = use_lds ? smem : gdirect;
+    for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) smem[i] = 0.f;
+    __syncthreads();
+    WaveMem w;
+    w.scr = smem + nacc + (int64_t)wave * pairs_wave_floats(NG);
+    w.xo = (int*)(w.scr + 64 * (NG + 1));
+    const int64_t nreg = (int64_t)NCLASS * A.sp_nw;
+    for (int64_t rg = (int64_t)blockIdx.x * W + wave; *A.sp_ovf == 0 && rg < nreg; rg += (int64_t)gridDim.x * W) {
+        const int ngr = min(A.sp_count[rg], (int)(A.sp_cap / 64));
+        for (int gi = 0; gi < ngr; gi++) {
+            const float* L = A.sp_list + rg * RW * A.sp_cap + (int64_t)gi * 64 + lane;
+            const int k = __float_as_int(L[0]);
+            float G[NG];
+#pragma unroll
+            for (int q = 0; q < NG; q++) G[q] = 0.f;
+            if (k >= 0) {
+                const int64_t ray = __float_as_int(L[A.sp_cap]);
+                const float t = L[2 * A.sp_cap];
+                const V3 o = ld3(A.origins + 3 * ray), d = ld3(A.dirs + 3 * ray), vd = ld3(A.viewdirs + 3 * ray);
+                V3 x = d * t + o;
+                if (A.coord_div != 1.f) x = v3(x.x / A.coord_div, x.y / A.coord_div, x.z / A.coord_div);
+                const float r = A.cone ? A.radii[ray] * t : A.radii[ray];
+                pair_grad<CID>(A, k, x, r, vd, L[3 * A.sp_cap], L[4 * A.sp_cap],
+                               v3(L[5 * A.sp_cap], L[6 * A.sp_cap], L[7 * A.sp_cap]), G);
+            }
+            // empty records only pad a region's last group, so the valid lanes lead it
+            const int nl = __popcll(__ballot(k >= 0));
+            reduce_runs<NG>(A, w, lane, nl, k >= 0 ? k : 0x7fffffff, G, dst, PG);
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        float* prow = partials + (int64_t)blockIdx.x * nacc;
+        for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) prow[i] = smem[i];
+    }
+}
+
+// MODE 0: gradient groups window by window; 1: the pending-pair queues (queues !=
+// nullptr), full groups of one family across rays; 2: the first kernel of the split
+// adjoint: rays whose hits were all kept and fit the pending list append their live
+// pairs' records to this wavefront's region of each family list instead of evaluating
+// them (the last group padded with empty records), the other rays are flagged for the
+// fused kernel.  With fbflag set, MODE 1 takes only the flagged rays (every ray after
+// an overflow).
+template <int NS, int CID, int MODE>
 __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, const float* __restrict__ g_rgb,
                                                                  const float* __restrict__ g_acc,
                                                                  const float* __restrict__ g_depth,
@@ -2395,8 +2466,10 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                                                                  float* __restrict__ gdirect,
                                                                  float* __restrict__ queues) {
     extern __shared__ float smem[];  // [S * PG] accumulators (use_lds) + per-wavefront regions
+    constexpr bool QUEUE = MODE != 0;
     constexpr int NQ = num_color_params(CID);
     constexpr int NG = 15 + NQ;  // gradient slots per pair
+    constexpr int NGL = MODE == 2 ? 3 : NG;  // LDS slots per lane (MODE 2: the live-pair codes only)
     constexpr int NPF = 5;       // per-pair values: alpha -> dL/dalpha, colour / dL/d(density, colour)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = blockDim.x >> 6;
@@ -2408,7 +2481,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     float* dst = use_lds ? smem : gdirect;
     float* prow = partials ? partials + (int64_t)blockIdx.x * nacc_max : nullptr;
     // this wavefront's pending-pair queues (nullptr: gradient groups run window by window)
-    float* queue = QUEUE ? queues + ((int64_t)blockIdx.x * W + wave) * ring_floats_per_wave() : nullptr;
+    float* queue = MODE == 1 ? queues + ((int64_t)blockIdx.x * W + wave) * ring_floats_per_wave() : nullptr;
     int qhead[NCLASS] = {0, 0, 0}, qtail[NCLASS] = {0, 0, 0};
     if (A0.scenes) {  // a batch: this block works on one scene's rays
         const int b = A0.block_base + (int)blockIdx.x;
@@ -2426,9 +2499,17 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     float* gacc = smem;
     for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) gacc[i] = 0.f;
     __syncthreads();
-    const WaveMem w = wave_mem(smem + nacc_max + wave * wave_mem_floats(NS, NPF, NG), NS, NPF, NG);
+    const WaveMem w = wave_mem(smem + nacc_max + wave * wave_mem_floats(NS, NPF, NGL), NS, NPF, NGL);
     const int64_t N = A.R * (int64_t)A.n;
+    const bool fb_all = MODE == 1 && A.fbflag && *A.sp_ovf != 0;
+    // MODE 2: this wavefront's list regions and how many records each holds
+    const int64_t wid = (int64_t)blockIdx.x * W + wave;
+    int cur[NCLASS] = {0, 0, 0};
+    bool full = false;
     for (int64_t ray = ray0; ray < ray_end; ray += ray_step) {
+        if constexpr (MODE == 1) {
+            if (A.fbflag && !fb_all && A.fbflag[ray] == 0) continue;  // done by the split adjoint
+        }
         RayConst rc = load_ray(A, ray);
         float ts[NS];
         bool valid[NS];
@@ -2570,6 +2651,12 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 preloaded = true;
             }
         }
+        if constexpr (MODE == 2) {
+            if (!preloaded) {  // left to the fused kernel
+                if (lane == 0) A.fbflag[ray] = 1;
+                continue;
+            }
+        }
         for (int ch = 0; ch < A.nch;) {
             int hoff = -1, kpoff = 0, knc = 0;
             if (preloaded) {
@@ -2612,7 +2699,12 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                     tot = nc > 0 ? __shfl(end, nc - 1, 64) : 0;
                     wave_sync();
                 } else {
-                    nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
+                    if constexpr (MODE != 2) {
+                        nc = append_hits<NS>(A, rc, w, sorted, ch * 64, lane, nh, np, tot);
+                    } else {
+                        nc = 0;
+                        tot = 0;
+                    }
                 }
                 nh += nc;
                 np += tot;
@@ -2620,7 +2712,9 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 do_flush = nh > HMAX - 64 || np >= A.flush_pairs || ch == A.nch;
             }
             if (!do_flush) continue;
-            if (!gkept) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
+            if constexpr (MODE != 2) {
+                if (!gkept) nh = refine_pending<NS>(A, rc, w, sorted, nh, lane, np);
+            }
             for (int c0 = 0; c0 < nh;) {
                 const int wbase = w.coff[c0];
                 const int c1 = window_end(w, c0, nh, wbase, lane);
@@ -2640,7 +2734,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 }
                 const int hc = c0 + lane;
                 const int hcls = hc < c1 ? (w.ccls[hc] & 3) : -1;
-                if (!gkept) {
+                if (MODE != 2 && !gkept) {
                   const int nx = class_sort_hits(w, hc, hcls, lane);
                   const int xt = wp;
                   for (int p0 = 0; p0 < xt; p0 += 64) {
@@ -2762,7 +2856,29 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
                 }
                 wave_sync();
                 const int nu = xs[NCLASS];
-                if constexpr (QUEUE) {
+                if constexpr (MODE == 2) {
+                    // append the live pairs class by class to this wavefront's regions
+#pragma unroll
+                    for (int xc = 0; xc < NCLASS; xc++) {
+                        const int nx = xs[xc + 1] - xs[xc];
+                        if (cur[xc] + nx > A.sp_cap) full = true;
+                        if (full) continue;
+                        float* L = A.sp_list + ((int64_t)xc * A.sp_nw + wid) * RW * A.sp_cap + cur[xc];
+                        for (int u = lane; u < nx; u += 64) {
+                            const int e = w.ul[xs[xc] + u];
+                            const int p = e & 0xffff, c = e >> 16;
+                            L[u] = __int_as_float(w.ck[c]);
+                            L[A.sp_cap + u] = __int_as_float((int)ray);
+                            L[2 * A.sp_cap + u] = w.ts[w.ci0[c] + (p - (w.coff[c] - wbase))];
+                            L[3 * A.sp_cap + u] = w.pb[p];
+                            L[4 * A.sp_cap + u] = w.pb[PBUF + p];
+                            L[5 * A.sp_cap + u] = w.pb[2 * PBUF + p];
+                            L[6 * A.sp_cap + u] = w.pb[3 * PBUF + p];
+                            L[7 * A.sp_cap + u] = w.pb[4 * PBUF + p];
+                        }
+                        cur[xc] += nx;
+                    }
+                } else if constexpr (QUEUE) {
                     // queue the live pairs class by class behind the pending ones; a class's
                     // full groups of 64 are evaluated as soon as they are there
 #pragma unroll
@@ -2893,7 +3009,16 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
             nh = np = 0;
         }
     }
-    if constexpr (QUEUE) {  // the partial groups left in the queues
+    if constexpr (MODE == 2) {  // pad each region to whole groups; a full region sends every ray to MODE 1
+#pragma unroll
+        for (int xc = 0; xc < NCLASS; xc++) {
+            const int padded = (cur[xc] + 63) / 64 * 64;  // <= sp_cap, a multiple of 64
+            float* L = A.sp_list + ((int64_t)xc * A.sp_nw + wid) * RW * A.sp_cap;
+            if (!full && cur[xc] + lane < padded) L[cur[xc] + lane] = __int_as_float(-1);
+            if (lane == 0) A.sp_count[(int64_t)xc * A.sp_nw + wid] = full ? 0 : padded / 64;
+        }
+        if (full && lane == 0) *A.sp_ovf = 1;
+    } else if constexpr (QUEUE) {  // the partial groups left in the queues
 #pragma unroll
         for (int xc = 0; xc < NCLASS; xc++)
             for (; qtail[xc] > qhead[xc]; qhead[xc] += 64)
@@ -2976,13 +3101,17 @@ static int g_max_lds = 0;
 // TUNE_BWD_QUEUE: the adjoint queues live pairs per SDF family across rays (1) or
 // evaluates each window's pairs as they come (0).  TUNE_CULL_GRID: the culling pass of
 // one scene tests only the strokes of its ray's grid cells (1) or every stroke (0).
+// TUNE_BWD_SPLIT: with kept hits, the adjoint runs as a per-ray kernel that lists the
+// live pairs' records and a kernel over the records (1), or as one kernel (0);
+// TUNE_SPLIT_CAP: list records per ray and SDF family (an overflow falls back to the one
+// kernel for every ray).
 enum {
     TUNE_FLUSH_PAIRS = 0, TUNE_BWD_OVERSUB = 1, TUNE_BATCH_OVERSUB = 2, TUNE_CULL_GROUP = 3, TUNE_BWD_QUEUE = 4,
-    TUNE_CULL_GRID = 5, TUNE_COUNT = 6
+    TUNE_CULL_GRID = 5, TUNE_BWD_SPLIT = 6, TUNE_SPLIT_CAP = 7, TUNE_COUNT = 8
 };
-static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1, 1, 1};
-static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1, 0, 0};
-static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16, 1, 1};
+static int g_tune[TUNE_COUNT] = {FLUSH_PAIRS, 1, 8, 1, 1, 1, 1, 32};
+static const int g_tune_lo[TUNE_COUNT] = {16, 1, 1, 1, 0, 0, 0, 1};
+static const int g_tune_hi[TUNE_COUNT] = {PBUF, 8, 64, 16, 1, 1, 1, 1024};
 
 typedef void (*FwdKernel)(RenderArgs, float*, float*, float*, float*, unsigned long long*);
 typedef void (*BwdKernel)(RenderArgs, const float*, const float*, const float*, const float*, float*, int, float*,
@@ -3010,13 +3139,15 @@ static FwdKernel fwd_kernel_ptr(int ns, int cid) {
     return tab[ns - 1][cid];
 }
 
-#define N3DS_BWD_ROW(NS, Q) \
-    {render_bwd_kernel<NS, 0, Q>, render_bwd_kernel<NS, 1, Q>, render_bwd_kernel<NS, 2, Q>, render_bwd_kernel<NS, 3, Q>}
-static BwdKernel bwd_kernel_ptr(int ns, int cid, bool queue = false) {
-    static const BwdKernel tab[2][4][4] = {
-        {N3DS_BWD_ROW(1, false), N3DS_BWD_ROW(2, false), N3DS_BWD_ROW(3, false), N3DS_BWD_ROW(4, false)},
-        {N3DS_BWD_ROW(1, true), N3DS_BWD_ROW(2, true), N3DS_BWD_ROW(3, true), N3DS_BWD_ROW(4, true)}};
-    return tab[queue ? 1 : 0][ns - 1][cid];
+#define N3DS_BWD_ROW(NS, M) \
+    {render_bwd_kernel<NS, 0, M>, render_bwd_kernel<NS, 1, M>, render_bwd_kernel<NS, 2, M>, render_bwd_kernel<NS, 3, M>}
+// mode 0: window by window, 1: queued groups, 2: the split adjoint's first kernel
+static BwdKernel bwd_kernel_ptr(int ns, int cid, int mode = 0) {
+    static const BwdKernel tab[3][4][4] = {
+        {N3DS_BWD_ROW(1, 0), N3DS_BWD_ROW(2, 0), N3DS_BWD_ROW(3, 0), N3DS_BWD_ROW(4, 0)},
+        {N3DS_BWD_ROW(1, 1), N3DS_BWD_ROW(2, 1), N3DS_BWD_ROW(3, 1), N3DS_BWD_ROW(4, 1)},
+        {N3DS_BWD_ROW(1, 2), N3DS_BWD_ROW(2, 2), N3DS_BWD_ROW(3, 2), N3DS_BWD_ROW(4, 2)}};
+    return tab[mode][ns - 1][cid];
 }
 
 static int fwd_lds_bytes(int ns, int cid) { return FWAVES * wave_mem_floats(ns, cid == 1 ? 4 : 1, 0) * 4; }
@@ -3129,6 +3260,12 @@ static RenderArgs make_args(const float* origins, const float* dirs, const float
     A.flush_pairs = g_tune[TUNE_FLUSH_PAIRS];
     A.cull_group = g_tune[TUNE_CULL_GROUP];
     A.ntask = 0;
+    A.sp_list = nullptr;
+    A.sp_count = nullptr;
+    A.sp_ovf = nullptr;
+    A.fbflag = nullptr;
+    A.sp_cap = 0;
+    A.sp_nw = 0;
     return A;
 }
 
@@ -3353,6 +3490,92 @@ N3DS_EXPORT int n3ds_render_forward(const float* origins, const float* direction
     return N3DS_OK;
 }
 
+// Split adjoint launch plan: the per-ray kernel (MODE 2, SPLIT_WAVES wavefronts per
+// block, no gradient rows), the pair kernel and the fused kernel for flagged rays (both
+// with per-block gradient rows in LDS), and the workspace pieces in floats: gradient
+// rows of both, the two kernels' queues, the family lists, then the counters and flags.
+constexpr int SPLIT_WAVES = 4;
+
+struct SplitPlan {
+    bool ok;
+    BwdKernel k1, kf;
+    int64_t nb1, nb2, nbf;
+    int64_t lds1, lds2;
+    BwdShape bsf;
+    int64_t cap;  // records per region (a multiple of 64)
+    int64_t off_rows, off_qf, off_lists, off_flags, total;
+};
+
+typedef void (*PairsKernel)(RenderArgs, float*, int, float*);
+
+static PairsKernel pairs_kernel_ptr(int cid) {
+    static const PairsKernel tab[4] = {render_bwd_pairs_kernel<0>, render_bwd_pairs_kernel<1>,
+                                       render_bwd_pairs_kernel<2>, render_bwd_pairs_kernel<3>};
+    return tab[cid];
+}
+
+static int split_plan(int ns, int cid, int64_t n_rays, int S, int PG, SplitPlan* sp) {
+    sp->ok = false;
+    if (!g_tune[TUNE_BWD_SPLIT] || !g_tune[TUNE_BWD_QUEUE]) return N3DS_OK;
+    sp->bsf = bwd_shape(ns, cid, S, PG);
+    if (!sp->bsf.use_lds || sp->bsf.lds_bytes > g_max_lds) return N3DS_OK;
+    const int ng = 15 + num_color_params(cid);
+    sp->lds2 = (int64_t)S * PG * 4 + (int64_t)PAIRS_WAVES * pairs_wave_floats(ng) * 4;
+    if (sp->lds2 > g_max_lds) return N3DS_OK;
+    sp->lds1 = (int64_t)SPLIT_WAVES * wave_mem_floats(ns, 5, 3) * 4;
+    sp->k1 = bwd_kernel_ptr(ns, cid, 2);
+    sp->kf = bwd_kernel_ptr(ns, cid, 1);
+    if (sp->lds1 > 65536)
+        N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)sp->k1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sp->lds1));
+    if (sp->lds2 > 65536)
+        N3DS_HIP_CHECK(hipFuncSetAttribute((const void*)pairs_kernel_ptr(cid),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)sp->lds2));
+    int per_cu = 0;
+    N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sp->k1, 64 * SPLIT_WAVES,
+                                                                (size_t)sp->lds1));
+    sp->nb1 = (int64_t)g_num_cu * (per_cu < 1 ? 1 : per_cu);
+    const int64_t need1 = (n_rays + SPLIT_WAVES - 1) / SPLIT_WAVES;
+    if (sp->nb1 > need1) sp->nb1 = need1;
+    per_cu = 0;
+    N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)pairs_kernel_ptr(cid),
+                                                                64 * PAIRS_WAVES, (size_t)sp->lds2));
+    sp->nb2 = (int64_t)g_num_cu * (per_cu < 1 ? 1 : per_cu);
+    if (bwd_blocks(sp->kf, sp->bsf, n_rays, &sp->nbf)) return -1;
+    const int64_t tot = (int64_t)S * PG;
+    // a MODE 2 wavefront takes rays wid, wid + nw, ...: at most rpw of them
+    const int64_t nw = sp->nb1 * SPLIT_WAVES, rpw = (n_rays + nw - 1) / nw;
+    sp->cap = (rpw * g_tune[TUNE_SPLIT_CAP] + 63) / 64 * 64;
+    sp->off_rows = 0;
+    sp->off_qf = (sp->nb2 + sp->nbf) * tot;
+    sp->off_lists = sp->off_qf + sp->nbf * sp->bsf.waves * ring_floats_per_wave();
+    sp->off_flags = sp->off_lists + (int64_t)NCLASS * nw * RW * sp->cap;
+    sp->total = sp->off_flags + (int64_t)NCLASS * nw + 1 + n_rays;
+    sp->ok = true;
+    return N3DS_OK;
+}
+
+// Floats of workspace the split adjoint needs (0: it does not apply); a caller that keeps
+// the forward's hits gives the backward max(this, n3ds_render_backward_workspace_floats).
+N3DS_EXPORT int64_t n3ds_render_backward_split_floats(int64_t n_rays, int n_strokes, int shape_width,
+                                                      int color_width) {
+    if (device_info()) return -1;
+    if (n_rays <= 0 || n_strokes <= 0) return 0;
+    const int PG = shape_width + color_width + 1;
+    int cid = -1;
+    for (int c = 0; c < 4; c++)
+        if (num_color_params(c) == color_width) cid = c;
+    if (cid < 0) return 0;
+    int64_t need = 0;
+    for (int ns = 1; ns <= 4; ns++) {
+        SplitPlan sp;
+        if (split_plan(ns, cid, n_rays, n_strokes, PG, &sp)) return -1;
+        if (!sp.ok) return 0;
+        if (sp.total > need) need = sp.total;
+    }
+    return need;
+}
+
 // Number of floats of device workspace the backward needs (per-block partial gradient rows).
 N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_strokes, int shape_width,
                                                           int color_width) {
@@ -3369,7 +3592,7 @@ N3DS_EXPORT int64_t n3ds_render_backward_workspace_floats(int64_t n_rays, int n_
         const BwdShape bs = bwd_shape(ns, cid, n_strokes, (int)PG);
         if (bs.lds_bytes > g_max_lds) continue;
         int64_t nb = 0;
-        if (bwd_blocks(bwd_kernel_ptr(ns, cid, g_tune[TUNE_BWD_QUEUE] != 0), bs, n_rays, &nb)) return -1;
+        if (bwd_blocks(bwd_kernel_ptr(ns, cid, g_tune[TUNE_BWD_QUEUE] != 0 ? 1 : 0), bs, n_rays, &nb)) return -1;
         int64_t n = bs.use_lds ? nb * tot : tot;
         if (g_tune[TUNE_BWD_QUEUE]) n += nb * bs.waves * ring_floats_per_wave();
         if (n > need) need = n;
@@ -3413,12 +3636,51 @@ static int render_backward_impl(const float* origins, const float* directions, c
         N3DS_REQUIRE(tot <= workspace_floats,
                      "%d strokes x %d gradient columns exceed the %lld-float scratch of the allocation-free backward",
                      n_strokes, PG, (long long)workspace_floats);
+    hipStream_t s = (hipStream_t)stream;
+    if (!fit_blocks && A.hdir && A.nch <= 64) {  // the split adjoint, when the workspace has room
+        SplitPlan sp;
+        st = split_plan(ns, color_id, n_rays, n_strokes, PG, &sp);
+        if (st) return st;
+        if (sp.ok && workspace_floats >= sp.total) {
+            float* rows = workspace + sp.off_rows;
+            const int64_t nw = sp.nb1 * SPLIT_WAVES;
+            A.sp_list = workspace + sp.off_lists;
+            A.sp_count = (int*)(workspace + sp.off_flags);
+            A.sp_ovf = A.sp_count + NCLASS * nw;
+            A.fbflag = A.sp_ovf + 1;
+            A.sp_cap = sp.cap;
+            A.sp_nw = (int)nw;
+            // the overflow flag and the ray flags start clear (every MODE 2 wavefront writes
+            // its own counts)
+            const int64_t nflag = 1 + n_rays;
+            hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((nflag + 255) / 256)), dim3(256), 0, s,
+                               (float*)A.sp_ovf, nflag);
+            N3DS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(sp.k1, dim3((unsigned)sp.nb1), dim3(64 * SPLIT_WAVES), (size_t)sp.lds1, s, A, grad_rgb,
+                               grad_acc, grad_depth, state, (float*)nullptr, 0, (float*)nullptr, (float*)nullptr);
+            N3DS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(pairs_kernel_ptr(color_id), dim3((unsigned)sp.nb2), dim3(64 * PAIRS_WAVES), (size_t)sp.lds2,
+                               s, A, rows, 1, (float*)nullptr);
+            N3DS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(sp.kf, dim3((unsigned)sp.nbf), dim3(64 * sp.bsf.waves), (size_t)sp.bsf.lds_bytes, s, A,
+                               grad_rgb, grad_acc, grad_depth, state, rows + sp.nb2 * tot, 1, (float*)nullptr,
+                               workspace + sp.off_qf);
+            N3DS_LAUNCH_CHECK();
+            const int nparts = (int)(sp.nb2 + sp.nbf);
+            hipLaunchKernelGGL(render_reduce_kernel,
+                               dim3((unsigned)((tot + 255) / 256), (unsigned)((nparts + RED_PARTS - 1) / RED_PARTS)),
+                               dim3(256), 0, s, rows, nparts, n_strokes, shape_width, color_width, grad_shape,
+                               grad_color, grad_density);
+            N3DS_LAUNCH_CHECK();
+            return N3DS_OK;
+        }
+    }
     // the pending-pair queues follow the gradient rows when the workspace has room for them
     bool queue = !fit_blocks && g_tune[TUNE_BWD_QUEUE] != 0;
     BwdKernel kern = nullptr;
     int64_t nb = 0, ws_need = 0;
     for (;;) {
-        kern = bwd_kernel_ptr(ns, color_id, queue);
+        kern = bwd_kernel_ptr(ns, color_id, queue ? 1 : 0);
         st = bwd_blocks(kern, bs, n_rays, &nb);
         if (st) return st;
         if (fit_blocks && bs.use_lds && nb * tot > workspace_floats) nb = workspace_floats / tot;
@@ -3429,7 +3691,6 @@ static int render_backward_impl(const float* origins, const float* directions, c
     N3DS_REQUIRE(workspace_floats >= ws_need, "workspace has %lld floats, need %lld", (long long)workspace_floats,
                  (long long)ws_need);
     float* queues = queue ? workspace + ws_need : nullptr;
-    hipStream_t s = (hipStream_t)stream;
     if (!bs.use_lds) {
         hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, workspace, tot);
         N3DS_LAUNCH_CHECK();

@@ -182,6 +182,9 @@ class _render_fn(Function):
         g_depth = None if g_depth is None else g_depth.float().contiguous()
         if scenes is None:
             nws = _lib.lib().n3ds_render_backward_workspace_floats(R, S, Pmax, Q)
+            if hd.numel() > 0 and nws >= 0:  # kept hits: room for the split adjoint
+                nsp = _lib.lib().n3ds_render_backward_split_floats(R, S, Pmax, Q)
+                nws = -1 if nsp < 0 else max(nws, nsp)
             bwd, batch_args = _lib.lib().n3ds_render_backward, ()
         else:
             nws = _lib.lib().n3ds_render_batch_backward_workspace_floats(scenes.host_ptr(), scenes.n_scenes, n1 - 1,

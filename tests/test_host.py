@@ -40,7 +40,7 @@ def _header_symbols():
 
 def test_header_and_signature_table_agree():
     syms = _header_symbols()
-    assert len(syms) == 35
+    assert len(syms) == 36
     assert syms == sorted(_lib.SIGNATURES)
 
 
@@ -157,9 +157,10 @@ def test_render_tuning_parameters():
     assert autotune.get('flush_pairs') == 192
     lib = _lib.lib()
     assert lib.n3ds_render_tuning(0, 5000) == -1 and b'outside' in lib.n3ds_last_error()
-    assert lib.n3ds_render_tuning(7, 1) == -1
+    assert lib.n3ds_render_tuning(8, 1) == -1
     assert old['bwd_queue'] == 1 and lib.n3ds_render_tuning(4, 2) == -1
     assert old['cull_grid'] == 1 and lib.n3ds_render_tuning(5, 2) == -1
+    assert old['bwd_split'] == 1 and old['split_cap'] == 32 and lib.n3ds_render_tuning(7, 0) == -1
     assert autotune.get('flush_pairs') == 192
     for k, v in old.items():
         autotune.set(k, v)

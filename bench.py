@@ -889,8 +889,8 @@ def main():
         tuning = autotune.current()
     elif args.autotune:
         from neural3dstrokes_amd import autotune
-        tuning, tuning_ms = autotune.tune(fused, names=('cull_grid', 'cull_group', 'flush_pairs', 'bwd_oversub',
-                                                        'bwd_queue'))
+        tuning, tuning_ms = autotune.tune(fused, names=('cull_grid', 'bwd_split', 'cull_group', 'flush_pairs',
+                                                        'bwd_oversub', 'bwd_queue'))
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]

@@ -114,7 +114,8 @@ struct RenderArgs {
     // split adjoint (render_bwd_kernel MODE 2, then render_bwd_pairs_kernel): per SDF
     // family and MODE 2 wavefront a field-major region of sp_cap adjoint records
     // (stroke, ray, depth, dL/dalpha, dL/d(density), dL/d(colour) x 3), sp_count [NCLASS,
-    // sp_nw] the 64-record groups each region holds, sp_ovf set when a region overflowed
+    // sp_nw] the 64-record groups each region holds (then their exclusive prefix and the
+    // total, group_scan_kernel), sp_ovf set when a region overflowed
     // (the fused kernel then takes every ray), fbflag [R] the rays left to the fused
     // kernel; nullptr outside the split adjoint
     float* sp_list;
@@ -2398,7 +2399,31 @@ __device__ __forceinline__ void queued_group(const RenderArgs& A, const WaveMem&
 }
 
 __host__ __device__ constexpr int pairs_wave_floats(int ng) { return 64 * (ng + 1) + 64; }
-constexpr int PAIRS_WAVES = 12;  // wavefronts per block of the pair kernel (one block per CU)
+
+// One block: c[0..n) (group counts) replaced by their exclusive prefix, the total in c[n].
+__global__ __launch_bounds__(1024) void group_scan_kernel(int* __restrict__ c, int64_t n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024, b0 = min((int64_t)t * per, n), b1 = min(b0 + per, n);
+    int sum = 0;
+    for (int64_t i = b0; i < b1; i++) sum += c[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the per-thread sums
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int64_t i = b0; i < b1; i++) {
+        const int v = c[i];
+        c[i] = run;
+        run += v;
+    }
+    if (t == 1023) c[n] = part[1023];
+}
+constexpr int PAIRS_WAVES = 16;  // wavefronts per block of the pair kernel at most (one block per CU)
 
 // Split adjoint, second kernel: the records of render_bwd_kernel<MODE 2>, 64 of one SDF
 // family per wavefront, through pair_grad; each stroke's run of lanes is summed into the
@@ -2419,11 +2444,27 @@ __global__ __launch_bounds__(64 * PAIRS_WAVES) void render_bwd_pairs_kernel(Rend
     WaveMem w;
     w.scr = smem + nacc + (int64_t)wave * pairs_wave_floats(NG);
     w.xo = (int*)(w.scr + 64 * (NG + 1));
+    // sp_count holds, after grid_scan_kernel, the groups before each region (and the total
+    // at index nreg); wavefront q takes groups [q G / Q, (q + 1) G / Q)
     const int64_t nreg = (int64_t)NCLASS * A.sp_nw;
-    for (int64_t rg = (int64_t)blockIdx.x * W + wave; *A.sp_ovf == 0 && rg < nreg; rg += (int64_t)gridDim.x * W) {
-        const int ngr = min(A.sp_count[rg], (int)(A.sp_cap / 64));
-        for (int gi = 0; gi < ngr; gi++) {
-            const float* L = A.sp_list + rg * RW * A.sp_cap + (int64_t)gi * 64 + lane;
+    const int64_t total = *A.sp_ovf == 0 ? A.sp_count[nreg] : 0;
+    const int64_t nq = (int64_t)gridDim.x * W, q = (int64_t)blockIdx.x * W + wave;
+    const int64_t g0 = total * q / nq, g1 = total * (q + 1) / nq;
+    int64_t rg = 0;
+    if (g0 < g1) {  // the region of group g0: the last one starting at or before it
+        int64_t lo = 0, hi = nreg - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (A.sp_count[mid] <= g0) lo = mid;
+            else hi = mid - 1;
+        }
+        rg = lo;
+    }
+    for (int64_t g = g0; g < g1; g++) {
+        while (A.sp_count[rg + 1] <= g) rg++;  // skip regions without groups
+        {
+            const int64_t gi = g - A.sp_count[rg];
+            const float* L = A.sp_list + rg * RW * A.sp_cap + gi * 64 + lane;
             const int k = __float_as_int(L[0]);
             float G[NG];
 #pragma unroll
@@ -2506,9 +2547,16 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     const int64_t wid = (int64_t)blockIdx.x * W + wave;
     int cur[NCLASS] = {0, 0, 0};
     bool full = false;
-    for (int64_t ray = ray0; ray < ray_end; ray += ray_step) {
+    unsigned long long fmask = 0ull;  // MODE 1 with fbflag: flags of this wavefront's next 64 rays
+    for (int64_t ray = ray0, it = 0; ray < ray_end; ray += ray_step, it++) {
         if constexpr (MODE == 1) {
-            if (A.fbflag && !fb_all && A.fbflag[ray] == 0) continue;  // done by the split adjoint
+            if (A.fbflag && !fb_all) {  // rays done by the split adjoint are skipped
+                if ((it & 63) == 0) {
+                    const int64_t mine = ray + lane * ray_step;
+                    fmask = __ballot(mine < ray_end && A.fbflag[mine] != 0);
+                }
+                if (!((fmask >> (it & 63)) & 1ull)) continue;
+            }
         }
         RayConst rc = load_ray(A, ray);
         float ts[NS];
@@ -3501,6 +3549,7 @@ struct SplitPlan {
     BwdKernel k1, kf;
     int64_t nb1, nb2, nbf;
     int64_t lds1, lds2;
+    int pw;  // wavefronts per block of the pair kernel
     BwdShape bsf;
     int64_t cap;  // records per region (a multiple of 64)
     int64_t off_rows, off_qf, off_lists, off_flags, total;
@@ -3520,8 +3569,12 @@ static int split_plan(int ns, int cid, int64_t n_rays, int S, int PG, SplitPlan*
     sp->bsf = bwd_shape(ns, cid, S, PG);
     if (!sp->bsf.use_lds || sp->bsf.lds_bytes > g_max_lds) return N3DS_OK;
     const int ng = 15 + num_color_params(cid);
-    sp->lds2 = (int64_t)S * PG * 4 + (int64_t)PAIRS_WAVES * pairs_wave_floats(ng) * 4;
-    if (sp->lds2 > g_max_lds) return N3DS_OK;
+    // the pair kernel: as many wavefronts per block as the rows leave LDS for
+    sp->pw = 0;
+    for (int wv = PAIRS_WAVES; wv >= 4 && !sp->pw; wv -= 4)
+        if ((int64_t)S * PG * 4 + (int64_t)wv * pairs_wave_floats(ng) * 4 <= g_max_lds) sp->pw = wv;
+    if (!sp->pw) return N3DS_OK;
+    sp->lds2 = (int64_t)S * PG * 4 + (int64_t)sp->pw * pairs_wave_floats(ng) * 4;
     sp->lds1 = (int64_t)SPLIT_WAVES * wave_mem_floats(ns, 5, 3) * 4;
     sp->k1 = bwd_kernel_ptr(ns, cid, 2);
     sp->kf = bwd_kernel_ptr(ns, cid, 1);
@@ -3539,7 +3592,7 @@ static int split_plan(int ns, int cid, int64_t n_rays, int S, int PG, SplitPlan*
     if (sp->nb1 > need1) sp->nb1 = need1;
     per_cu = 0;
     N3DS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)pairs_kernel_ptr(cid),
-                                                                64 * PAIRS_WAVES, (size_t)sp->lds2));
+                                                                64 * sp->pw, (size_t)sp->lds2));
     sp->nb2 = (int64_t)g_num_cu * (per_cu < 1 ? 1 : per_cu);
     if (bwd_blocks(sp->kf, sp->bsf, n_rays, &sp->nbf)) return -1;
     const int64_t tot = (int64_t)S * PG;
@@ -3550,7 +3603,7 @@ static int split_plan(int ns, int cid, int64_t n_rays, int S, int PG, SplitPlan*
     sp->off_qf = (sp->nb2 + sp->nbf) * tot;
     sp->off_lists = sp->off_qf + sp->nbf * sp->bsf.waves * ring_floats_per_wave();
     sp->off_flags = sp->off_lists + (int64_t)NCLASS * nw * RW * sp->cap;
-    sp->total = sp->off_flags + (int64_t)NCLASS * nw + 1 + n_rays;
+    sp->total = sp->off_flags + ((int64_t)NCLASS * nw + 1) + 1 + n_rays;  // counts + total, overflow, ray flags
     sp->ok = true;
     return N3DS_OK;
 }
@@ -3646,7 +3699,7 @@ static int render_backward_impl(const float* origins, const float* directions, c
             const int64_t nw = sp.nb1 * SPLIT_WAVES;
             A.sp_list = workspace + sp.off_lists;
             A.sp_count = (int*)(workspace + sp.off_flags);
-            A.sp_ovf = A.sp_count + NCLASS * nw;
+            A.sp_ovf = A.sp_count + NCLASS * nw + 1;
             A.fbflag = A.sp_ovf + 1;
             A.sp_cap = sp.cap;
             A.sp_nw = (int)nw;
@@ -3659,7 +3712,9 @@ static int render_backward_impl(const float* origins, const float* directions, c
             hipLaunchKernelGGL(sp.k1, dim3((unsigned)sp.nb1), dim3(64 * SPLIT_WAVES), (size_t)sp.lds1, s, A, grad_rgb,
                                grad_acc, grad_depth, state, (float*)nullptr, 0, (float*)nullptr, (float*)nullptr);
             N3DS_LAUNCH_CHECK();
-            hipLaunchKernelGGL(pairs_kernel_ptr(color_id), dim3((unsigned)sp.nb2), dim3(64 * PAIRS_WAVES), (size_t)sp.lds2,
+            hipLaunchKernelGGL(group_scan_kernel, dim3(1), dim3(1024), 0, s, A.sp_count, (int64_t)NCLASS * nw);
+            N3DS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(pairs_kernel_ptr(color_id), dim3((unsigned)sp.nb2), dim3(64 * sp.pw), (size_t)sp.lds2,
                                s, A, rows, 1, (float*)nullptr);
             N3DS_LAUNCH_CHECK();
             hipLaunchKernelGGL(sp.kf, dim3((unsigned)sp.nbf), dim3(64 * sp.bsf.waves), (size_t)sp.bsf.lds_bytes, s, A,

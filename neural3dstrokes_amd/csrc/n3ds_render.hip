@@ -1216,8 +1216,7 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
 constexpr int GRID_MAX_NCH = 64;
 constexpr int64_t GRID_WORDS = 1 << 19;  // cell-mask words (64-bit) at most, 4 MiB
 constexpr int GRID_RES_MAX = 32;
-constexpr int GRID_RAY_WAVES = 4;        // wavefronts per block of grid_rays_kernel
-constexpr int GRID_RAYS_PER_WAVE = 4;    // rays per wavefront of grid_rays_kernel
+constexpr int GRID_RAYS_PER_BLOCK = 256;  // rays (threads) per block of grid_rays_kernel
 constexpr int GRID_SETUP_THREADS = 1024;
 constexpr int GRID_CAP = 128;            // candidates per culling batch
 constexpr int GRID_STRIDE = 64;          // chunks per culling task at most (one task per ray)
@@ -1261,86 +1260,33 @@ __device__ __forceinline__ bool grid_item_box(const float* __restrict__ rec, int
     return true;
 }
 
-// Per ray: the segment [tlo, thi] and margin m that the culling pass derives from the
-// sample depths (load_ray, or unsorted_range when the depths are not sorted), and the
-// block's largest margin; the whole launch also clears the cell masks.  A wavefront
-// takes GRID_RAYS_PER_WAVE rays, their depths loaded together.
-
-template <int NS>
-__global__ __launch_bounds__(64 * GRID_RAY_WAVES) void grid_rays_kernel(RenderArgs A, float* __restrict__ seg,
-                                                                         float* __restrict__ part,
-                                                                         unsigned long long* __restrict__ cells,
-                                                                         int64_t ncell_words) {
+// Per ray: the segment [tlo, thi] and margin m that the culling pass derives from sorted
+// sample depths (load_ray: the first and last interval midpoints), and the block's
+// largest margin; the whole launch also clears the cell masks.  A ray whose depths are
+// not sorted ignores its mask (render_cull_grid_kernel tests it against every stroke),
+// so its segment here need not bound its samples.
+__global__ __launch_bounds__(256) void grid_rays_kernel(RenderArgs A, float* __restrict__ seg, float* __restrict__ part,
+                                                        unsigned long long* __restrict__ cells, int64_t ncell_words) {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, gn = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = gt; i < ncell_words; i += gn) cells[i] = 0ull;
-    __shared__ float bm[GRID_RAY_WAVES];
+    __shared__ float bm[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t ray0 = ((int64_t)blockIdx.x * GRID_RAY_WAVES + wave) * GRID_RAYS_PER_WAVE;
-    float tm[GRID_RAYS_PER_WAVE][NS], tn[GRID_RAYS_PER_WAVE][NS], rad[GRID_RAYS_PER_WAVE];
-#pragma unroll
-    for (int r = 0; r < GRID_RAYS_PER_WAVE; r++) {
-        const int64_t ray = ray0 + r;
-        rad[r] = A.radii[ray < A.R ? ray : 0];
-        const float* td = A.tdist + (ray < A.R ? ray : 0) * (A.n + 1);
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const int i = lane + 64 * s;
-            const bool ok = ray < A.R && i < A.n;
-            tm[r][s] = ok ? (td[i] + td[i + 1]) / 2.f : 0.f;
-            tn[r][s] = ok && i + 1 < A.n ? (td[i + 1] + td[i + 2]) / 2.f : 0.f;
-        }
-    }
-    float mmax = 0.f;
-#pragma unroll
-    for (int r = 0; r < GRID_RAYS_PER_WAVE; r++) {
-        const int64_t ray = ray0 + r;
-        if (ray >= A.R) break;
-        float lo = INFINITY, hi = -INFINITY;
-        bool bad = false;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const int i = lane + 64 * s;
-            if (i < A.n) {
-                lo = fminf(lo, tm[r][s]);
-                hi = fmaxf(hi, tm[r][s]);
-                if (i + 1 < A.n && tm[r][s] > tn[r][s]) bad = true;
-            }
-        }
-        float tlo, thi;
-        if (__ballot(bad) == 0ull) {
-            // the first and last midpoints, (td[i] + td[i + 1]) / 2 as 0.5 (td[i] + td[i + 1])
-            tlo = __shfl(tm[r][0], 0, 64);
-            const int il = A.n - 1;
-            thi = 0.f;
-#pragma unroll
-            for (int s = 0; s < NS; s++)
-                if (s == il / 64) thi = __shfl(tm[r][s], il % 64, 64);
-        } else {
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                lo = fminf(lo, __shfl_xor(lo, off, 64));
-                hi = fmaxf(hi, __shfl_xor(hi, off, 64));
-            }
-            tlo = lo;
-            thi = hi;
-        }
-        const float radii = rad[r];
+    float m = 0.f;
+    if (gt < A.R) {
+        const float* td = A.tdist + gt * (A.n + 1);
+        const float tlo = 0.5f * (td[0] + td[1]), thi = 0.5f * (td[A.n - 1] + td[A.n]);
+        const float radii = A.radii[gt];
         const float rmax = A.cone ? radii * fmaxf(fabsf(tlo), fabsf(thi)) : radii;
-        const float m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
-        mmax = fmaxf(mmax, m);
-        if (lane == 0) {
-            seg[ray * 3] = tlo;
-            seg[ray * 3 + 1] = thi;
-            seg[ray * 3 + 2] = m;
-        }
+        m = A.delta > 0.f ? A.delta * rmax * A.tail : 1e-6f;
+        seg[gt * 3] = tlo;
+        seg[gt * 3 + 1] = thi;
+        seg[gt * 3 + 2] = m;
     }
-    if (lane == 0) bm[wave] = mmax;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane == 0) bm[wave] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float b = 0.f;
-        for (int q = 0; q < GRID_RAY_WAVES; q++) b = fmaxf(b, bm[q]);
-        part[blockIdx.x] = b;
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(bm[0], bm[1]), fmaxf(bm[2], bm[3]));
 }
 
 // One block: the largest ray margin, the box of every stroke's (piece's) bound grown
@@ -1640,8 +1586,12 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_grid_kernel(RenderArg
     const bool sorted = stage_depths<NS>(A, ray, lane, w, ts, valid);
     if (!sorted) unsorted_range<NS>(A, rc, ts, valid);
     const int nch = A.nch;
-    // lane j: the mask of chunk j and the candidates before it
-    const unsigned long long mj = lane < nch ? rmask[ray * nch + lane] : 0ull;
+    // lane j: the mask of chunk j (every stroke of it when the depths are not sorted) and
+    // the candidates before it
+    const int nvalid = min(64, A.S - 64 * lane);
+    const unsigned long long mj =
+        lane >= nch ? 0ull
+                    : (sorted ? rmask[ray * nch + lane] : (nvalid >= 64 ? ~0ull : ((1ull << nvalid) - 1ull)));
     const int pc = __popcll(mj);
     const int pincl = wave_incl_scan(pc, lane);
     if (lane < nch) {
@@ -2400,10 +2350,13 @@ __device__ __forceinline__ void queued_group(const RenderArgs& A, const WaveMem&
 
 __host__ __device__ constexpr int pairs_wave_floats(int ng) { return 64 * (ng + 1) + 64; }
 
-// One block: c[0..n) (group counts) replaced by their exclusive prefix, the total in c[n].
-__global__ __launch_bounds__(1024) void group_scan_kernel(int* __restrict__ c, int64_t n) {
+// One block: c[0..n) (group counts) replaced by their exclusive prefix, the total in c[n];
+// also clears z[0..nz).
+__global__ __launch_bounds__(1024) void group_scan_kernel(int* __restrict__ c, int64_t n, float* __restrict__ z,
+                                                          int64_t nz) {
     __shared__ int part[1024];
     const int t = threadIdx.x;
+    for (int64_t i = t; i < nz; i += 1024) z[i] = 0.f;
     const int64_t per = (n + 1023) / 1024, b0 = min((int64_t)t * per, n), b1 = min(b0 + per, n);
     int sum = 0;
     for (int64_t i = b0; i < b1; i++) sum += c[i];
@@ -3397,7 +3350,7 @@ static void grid_offsets(int64_t n_rays, int nch, int64_t* off) {
     const int res = grid_res(nch);
     off[0] = 0;
     off[1] = off[0] + align256(GP_COUNT * 4);
-    off[2] = off[1] + align256((n_rays + GRID_RAY_WAVES * GRID_RAYS_PER_WAVE - 1) / (GRID_RAY_WAVES * GRID_RAYS_PER_WAVE) * 4);
+    off[2] = off[1] + align256((n_rays + GRID_RAYS_PER_BLOCK - 1) / GRID_RAYS_PER_BLOCK * 4);
     off[3] = off[2] + align256(n_rays * 12);
     off[4] = off[3] + align256((int64_t)res * res * res * nch * 8);
     off[5] = off[4] + align256(n_rays * nch * 8);
@@ -3408,7 +3361,7 @@ static GridWs grid_layout(char* base, int64_t n_rays, int nch) {
     int64_t off[6];
     grid_offsets(n_rays, nch, off);
     g.res = grid_res(nch);
-    g.nparts = (n_rays + GRID_RAY_WAVES * GRID_RAYS_PER_WAVE - 1) / (GRID_RAY_WAVES * GRID_RAYS_PER_WAVE);
+    g.nparts = (n_rays + GRID_RAYS_PER_BLOCK - 1) / GRID_RAYS_PER_BLOCK;
     g.ncell_words = (int64_t)g.res * g.res * g.res * nch;
     g.gp = (float*)(base + off[0]);
     g.part = (float*)(base + off[1]);
@@ -3427,7 +3380,6 @@ static int64_t grid_workspace_bytes(int64_t n_rays, int nch) {
 }
 
 typedef void (*CullGridKernel)(RenderArgs, const unsigned long long*, int*, int*);
-typedef void (*GridRaysKernel)(RenderArgs, float*, float*, unsigned long long*, int64_t);
 
 // The culling pass of one scene with the grid: rays, setup, insertion, walk, then the
 // candidate culling.  Launches nothing (and leaves *used false) when the grid does not
@@ -3444,11 +3396,9 @@ static int launch_cull_grid(RenderArgs& A, int ns, void* workspace, int64_t work
     int* counts = (int*)workspace;
     int* slots = counts + A.R * A.nch_stride;
     const GridWs g = grid_layout((char*)workspace + cb, A.R, A.nch);
-    static const GridRaysKernel rk[4] = {grid_rays_kernel<1>, grid_rays_kernel<2>, grid_rays_kernel<3>,
-                                         grid_rays_kernel<4>};
     static const CullGridKernel ck[4] = {render_cull_grid_kernel<1>, render_cull_grid_kernel<2>,
                                          render_cull_grid_kernel<3>, render_cull_grid_kernel<4>};
-    hipLaunchKernelGGL(rk[ns - 1], dim3((unsigned)g.nparts), dim3(64 * GRID_RAY_WAVES), 0, s, A, g.seg, g.part,
+    hipLaunchKernelGGL(grid_rays_kernel, dim3((unsigned)g.nparts), dim3(GRID_RAYS_PER_BLOCK), 0, s, A, g.seg, g.part,
                        g.cells, g.ncell_words);
     N3DS_LAUNCH_CHECK();
     hipLaunchKernelGGL(grid_setup_kernel, dim3(1), dim3(GRID_SETUP_THREADS), 0, s, A, (const float*)g.part,
@@ -3566,8 +3516,11 @@ static PairsKernel pairs_kernel_ptr(int cid) {
 static int split_plan(int ns, int cid, int64_t n_rays, int S, int PG, SplitPlan* sp) {
     sp->ok = false;
     if (!g_tune[TUNE_BWD_SPLIT] || !g_tune[TUNE_BWD_QUEUE]) return N3DS_OK;
-    sp->bsf = bwd_shape(ns, cid, S, PG);
-    if (!sp->bsf.use_lds || sp->bsf.lds_bytes > g_max_lds) return N3DS_OK;
+    // the one-kernel adjoint for the flagged rays adds into one global row set
+    sp->bsf.waves = BWAVES;
+    sp->bsf.use_lds = 0;
+    sp->bsf.lds_bytes = (int64_t)BWAVES * bwd_wave_floats(ns, cid) * 4;
+    if (sp->bsf.lds_bytes > g_max_lds) return N3DS_OK;
     const int ng = 15 + num_color_params(cid);
     // the pair kernel: as many wavefronts per block as the rows leave LDS for
     sp->pw = 0;
@@ -3600,7 +3553,7 @@ static int split_plan(int ns, int cid, int64_t n_rays, int S, int PG, SplitPlan*
     const int64_t nw = sp->nb1 * SPLIT_WAVES, rpw = (n_rays + nw - 1) / nw;
     sp->cap = (rpw * g_tune[TUNE_SPLIT_CAP] + 63) / 64 * 64;
     sp->off_rows = 0;
-    sp->off_qf = (sp->nb2 + sp->nbf) * tot;
+    sp->off_qf = (sp->nb2 + 1) * tot;
     sp->off_lists = sp->off_qf + sp->nbf * sp->bsf.waves * ring_floats_per_wave();
     sp->off_flags = sp->off_lists + (int64_t)NCLASS * nw * RW * sp->cap;
     sp->total = sp->off_flags + ((int64_t)NCLASS * nw + 1) + 1 + n_rays;  // counts + total, overflow, ray flags
@@ -3712,16 +3665,17 @@ static int render_backward_impl(const float* origins, const float* directions, c
             hipLaunchKernelGGL(sp.k1, dim3((unsigned)sp.nb1), dim3(64 * SPLIT_WAVES), (size_t)sp.lds1, s, A, grad_rgb,
                                grad_acc, grad_depth, state, (float*)nullptr, 0, (float*)nullptr, (float*)nullptr);
             N3DS_LAUNCH_CHECK();
-            hipLaunchKernelGGL(group_scan_kernel, dim3(1), dim3(1024), 0, s, A.sp_count, (int64_t)NCLASS * nw);
+            hipLaunchKernelGGL(group_scan_kernel, dim3(1), dim3(1024), 0, s, A.sp_count, (int64_t)NCLASS * nw,
+                               rows + sp.nb2 * tot, tot);
             N3DS_LAUNCH_CHECK();
             hipLaunchKernelGGL(pairs_kernel_ptr(color_id), dim3((unsigned)sp.nb2), dim3(64 * sp.pw), (size_t)sp.lds2,
                                s, A, rows, 1, (float*)nullptr);
             N3DS_LAUNCH_CHECK();
             hipLaunchKernelGGL(sp.kf, dim3((unsigned)sp.nbf), dim3(64 * sp.bsf.waves), (size_t)sp.bsf.lds_bytes, s, A,
-                               grad_rgb, grad_acc, grad_depth, state, rows + sp.nb2 * tot, 1, (float*)nullptr,
+                               grad_rgb, grad_acc, grad_depth, state, (float*)nullptr, 0, rows + sp.nb2 * tot,
                                workspace + sp.off_qf);
             N3DS_LAUNCH_CHECK();
-            const int nparts = (int)(sp.nb2 + sp.nbf);
+            const int nparts = (int)(sp.nb2 + 1);
             hipLaunchKernelGGL(render_reduce_kernel,
                                dim3((unsigned)((tot + 255) / 256), (unsigned)((nparts + RED_PARTS - 1) / RED_PARTS)),
                                dim3(256), 0, s, rows, nparts, n_strokes, shape_width, color_width, grad_shape,

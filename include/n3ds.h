@@ -209,9 +209,12 @@ int n3ds_render_prepare(float* table, const int32_t* stroke_ids, const float* sh
  * kept chunk by chunk (64 strokes) until the ray's capacity is used up; the
  * backward recomputes the chunks that did not fit.
  * workspace (device, n3ds_render_forward_workspace_bytes; or NULL): when
- * given, culling runs first as its own pass, one wavefront per (ray, chunk of
- * 64 strokes), and the forward reads its hits; when NULL the forward culls
- * as it goes (no extra memory).  Results are identical. */
+ * given, culling runs first as its own pass and the forward reads its hits;
+ * for up to 64 chunks of 64 strokes the pass first bins the strokes in a
+ * uniform grid (per cell and chunk, the strokes whose culling bound overlaps
+ * the cell) and tests each ray only against the strokes of the cells its
+ * segment crosses (n3ds_render_tuning parameter 5); when NULL the forward
+ * culls as it goes (no extra memory).  Results are identical. */
 int n3ds_render_forward(const float* origins, const float* directions, const float* viewdirs, const float* radii,
                         const float* tdist, const float* table, const float* color_params,
                         const float* density_params, int64_t n_rays, int n_samples, int n_strokes,
@@ -243,14 +246,21 @@ int n3ds_render_extras(const float* state, const float* tdist, const float* dire
  * 128), 1 = backward persistent blocks per resident block (1..8, default 1),
  * 2 = batch backward blocks per resident block (1..64, default 8; read by
  * n3ds_render_batch_plan), 3 = 64-stroke chunks per wavefront of the culling
- * pass (1..16, default 1; a batch's plan depends on it).  value < 0 only
+ * pass (1..16, default 1; a batch's plan depends on it), 4 = the adjoint's
+ * gradient groups queued per SDF family across rays (0..1, default 1), 5 = the
+ * culling pass of one scene with the stroke grid (0..1, default 1), 6 = the
+ * split adjoint when its workspace is given (0..1, default 1), 7 = records per
+ * ray and SDF family the split adjoint's lists hold (1..1024, default 32; a
+ * fuller list sends every ray to the one-kernel adjoint).  value < 0 only
  * queries.  Returns the previous
  * value, or -1 for an unknown parameter or a value out of range.  Results do
  * not depend on these parameters beyond float summation order. */
 int n3ds_render_tuning(int param, int value);
 
 /* Bytes of the forward's culling-pass workspace: 260 B per (ray, chunk of 64
- * strokes); max_strokes = the stroke count (a batch: its largest scene's). */
+ * strokes), plus, for at most 64 chunks, the stroke grid (at most 4 MiB of cell
+ * masks, 12 B per ray and 8 B per (ray, chunk)); max_strokes = the stroke count
+ * (a batch: its largest scene's). */
 int64_t n3ds_render_forward_workspace_bytes(int64_t n_rays, int max_strokes);
 
 /* Floats of device workspace n3ds_render_backward needs (queries the current

@@ -459,12 +459,11 @@ __device__ __forceinline__ Pair eval_pair(const float* __restrict__ rec, int bas
     if (curve) {
         q.p = x;
     } else {
-        const float* m = rec + R_ROT;
+        // unit-space point diag(1/s) R^T (x - t) through the stored map (no divisions)
+        const float* m = rec + R_M;
         V3 d = x - ld3(rec + R_TR);
-        V3 pr = v3(m[0] * d.x + m[3] * d.y + m[6] * d.z, m[1] * d.x + m[4] * d.y + m[7] * d.z,
-                   m[2] * d.x + m[5] * d.y + m[8] * d.z);
-        V3 s = ld3(rec + R_SC);
-        q.p = v3(pr.x / s.x, pr.y / s.y, pr.z / s.z);
+        q.p = v3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[3] * d.x + m[4] * d.y + m[5] * d.z,
+                 m[6] * d.x + m[7] * d.y + m[8] * d.z);
     }
     q.sdf = sdf_eval(base, q.p, prm);
     q.fac = 1.f;

@@ -306,7 +306,7 @@ template <bool POWER>
 __device__ __forceinline__ CurvePick curve_pick(V3 pos, const V3 K[4]) {
     constexpr float step = 1.0f / CURVE_SEGMENTS;
     CurvePick P;
-    float dist_min = 1e9f;
+    float dist_min = 1e18f;  // squared: the reference's distance bound of 1e9
     P.t = 0.f; P.seg = 0; P.s = 0.f; P.sraw = 0.f; P.num = 0.f; P.den = 1.f;
     P.S0 = v3s(0.f); P.AB = v3s(0.f);
     float t_start = 0.f;
@@ -318,10 +318,12 @@ __device__ __forceinline__ CurvePick curve_pick(V3 pos, const V3 K[4]) {
         V3 ab = s1 - s0;
         float num = dot(pos - s0, ab);
         float den = dot(ab, ab);
-        float sr = num / den;
+        float sr = num * __builtin_amdgcn_rcpf(den);
         float s = clampf(sr, 0.f, 1.f);
         V3 pp = s0 + s * ab;
-        float dist = safe_len(pos - pp);
+        // squared distances order the segments as the distances do (sqrt is monotonic)
+        V3 dv = pos - pp;
+        float dist = dot(dv, dv);
         if (dist < dist_min) {
             dist_min = dist;
             P.t = t_start * (1.f - s) + t_end * s;

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_fwd_kernel(float* __restr
                                                                const float* __restrict__ colors,
                                                                const float* __restrict__ dp, int64_t N, int64_t S,
                                                                int64_t ppb) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t n0 = (int64_t)blockIdx.x * ppb, n1 = min(N, n0 + ppb);
     for (int64_t n = n0 + wave; n < n1; n += CWAVES) {
         float T = 1.f, Dn = 0.f, C[D];
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64 * CWAVES) void compose_bwd_kernel(
     // walked back to front (pass 2); G = 1 up to MAXCH chunks, so pass 2 recomputes nothing
     __shared__ float t_in[CWAVES][MAXCH];
     __shared__ float t_grp[CWAVES][MAXCH];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const bool lds = S <= LDS_GDP_MAX;
     if (lds)
         for (int64_t i = threadIdx.x; i < S; i += 64 * CWAVES) lds_gdp[i] = 0.f;

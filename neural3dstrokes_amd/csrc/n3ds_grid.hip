@@ -774,7 +774,7 @@ __global__ __launch_bounds__(PF_BWD_THREADS) void prop_bwd_kernel(PropArgs P, co
     __shared__ float sdh[NW][64][PF_CH + 1], sh[NW][64][PF_CH + 1];            // one chunk of hidden units
     __shared__ float sg[NW][64];
     const int F = FX > 0 ? FX : P.L * C;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     for (int i = threadIdx.x; i < PF_HID * F; i += PF_BWD_THREADS) {
         sW1[i] = P.W1[i];
         aW1[i] = 0.f;

@@ -1032,7 +1032,7 @@ template <int NS>
 __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0, int* __restrict__ counts,
                                                                   int* __restrict__ slots) {
     extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t t = (int64_t)blockIdx.x * CWAVES + wave;
     if (t >= A0.ntask) return;
     // a task culls cull_group consecutive chunks of one ray; ray-major, so neighbouring
@@ -1269,7 +1269,7 @@ __global__ __launch_bounds__(256) void grid_rays_kernel(RenderArgs A, float* __r
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, gn = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = gt; i < ncell_words; i += gn) cells[i] = 0ull;
     __shared__ float bm[4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     float m = 0.f;
     if (gt < A.R) {
         const float* td = A.tdist + gt * (A.n + 1);
@@ -1294,7 +1294,7 @@ __global__ __launch_bounds__(256) void grid_rays_kernel(RenderArgs A, float* __r
 __global__ __launch_bounds__(GRID_SETUP_THREADS) void grid_setup_kernel(RenderArgs A, const float* __restrict__ part,
                                                                         int nparts, int res, float* __restrict__ gp) {
     __shared__ float red[GRID_SETUP_THREADS / 64][8];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     float m = 0.f;
     bool bad = false;
 #pragma unroll 4
@@ -1383,7 +1383,7 @@ __device__ __forceinline__ int grid_cell_of(float x, float lo, float icell, int 
 __global__ __launch_bounds__(256) void grid_insert_kernel(RenderArgs A, const float* __restrict__ gp,
                                                           unsigned long long* __restrict__ cells) {
     const int lane = threadIdx.x & 63;
-    const int64_t it = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t it = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (it >= (int64_t)A.S * NPIECE || gp[GP_ALL] != 0.f) return;
     const int k = (int)(it / NPIECE), piece = (int)(it % NPIECE);
     V3 lo, hi;
@@ -1426,7 +1426,7 @@ __global__ __launch_bounds__(64 * GRID_WALK_WAVES) void grid_walk_kernel(RenderA
     static_assert(GRID_WALK_RAYS * GRID_WALK_PIECES == 64 && GRID_WALK_CH * 8 == 64, "lane maps");
     __shared__ int lists[GRID_WALK_WAVES][GRID_WALK_RAYS][GRID_WALK_LIST];
     __shared__ int nlist[GRID_WALK_WAVES][GRID_WALK_RAYS];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int ngrp = (A.nch + GRID_WALK_CH - 1) / GRID_WALK_CH;
     const int64_t w = (int64_t)blockIdx.x * GRID_WALK_WAVES + wave;
     const int64_t nblk = (A.R + GRID_WALK_RAYS - 1) / GRID_WALK_RAYS;
@@ -1565,7 +1565,7 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_grid_kernel(RenderArg
                                                                        int* __restrict__ counts,
                                                                        int* __restrict__ slots) {
     extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t ray = (int64_t)blockIdx.x * CWAVES + wave;
     if (ray >= A.R) return;
     WaveMem w;
@@ -1778,7 +1778,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
                                                 unsigned long long* __restrict__ counters) {
     extern __shared__ float smem[];
     constexpr int NPF = CID == 1 ? 4 : 1;  // alpha (+ colour for gradient_rgb)
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t ray = (int64_t)blockIdx.x * FWAVES + wave;
     if (ray >= A0.R) return;
     RenderArgs A = A0;
@@ -1798,7 +1798,6 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
     V3 C[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) { T[s] = 1.f; U[s] = 0.f; Dn[s] = 0.f; C[s] = v3s(0.f); }
-    unsigned long long my_pos = 0, my_ev = 0, my_vis = 0;  // lane b counts stroke type b (counting pass)
     // cursors into this ray's region of the kept hits / pair values; once a group of
     // chunks does not fit, it and every later chunk are left to the adjoint to recompute
     int hoff = 0, poff = 0;
@@ -1990,18 +1989,19 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
                 }
                 if (counters) {
                     const int base = ibits(A.table[(int64_t)k * REC + R_BASE]);
-                    if (lane == base) { my_pos += tp; my_ev += cn; my_vis += 1; }
+                    // counting pass only: per stroke type, pairs with alpha > 0, evaluated
+                    // pairs and hits (atomics, so no registers stay live in the timed path)
+                    if (lane == 0) {
+                        atomicAdd(counters + base, (unsigned long long)tp);
+                        atomicAdd(counters + NUM_BASE + base, (unsigned long long)cn);
+                        atomicAdd(counters + 2 * NUM_BASE, 1ull);
+                    }
                 }
             }
             wave_sync();
             c0 = c1;
         }
         nh = np = kept_np = 0;
-    }
-    if (counters && lane < NUM_BASE) {
-        if (my_pos) atomicAdd(counters + lane, my_pos);
-        if (my_ev) atomicAdd(counters + NUM_BASE + lane, my_ev);
-        if (my_vis) atomicAdd(counters + 2 * NUM_BASE, my_vis);
     }
     // save per-sample composite state and run the ray march
     const int64_t N = A.R * (int64_t)A.n;
@@ -2087,7 +2087,7 @@ __global__ __launch_bounds__(64 * XWAVES) void render_extras_kernel(const float*
                                                                     const float* __restrict__ dirs,
                                                                     const float* __restrict__ t_far, int64_t R,
                                                                     int n, int opaque, float* __restrict__ out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t ray = (int64_t)blockIdx.x * XWAVES + wave;
     if (ray >= R) return;
     const float* td = tdist + ray * (n + 1);
@@ -2386,7 +2386,7 @@ __global__ __launch_bounds__(64 * PAIRS_WAVES) void render_bwd_pairs_kernel(Rend
                                                                        int use_lds, float* __restrict__ gdirect) {
     extern __shared__ float smem[];  // [S * PG] accumulators (use_lds) + per-wavefront regions
     constexpr int NG = 15 + num_color_params(CID);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int W = blockDim.x >> 6;
     const int PG = A.Pmax + A.Q + 1;
     const int64_t nacc = use_lds ? (int64_t)A.S * PG : 0;
@@ -2464,7 +2464,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
     constexpr int NG = 15 + NQ;  // gradient slots per pair
     constexpr int NGL = MODE == 2 ? 3 : NG;  // LDS slots per lane (MODE 2: the live-pair codes only)
     constexpr int NPF = 5;       // per-pair values: alpha -> dL/dalpha, colour / dL/d(density, colour)
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int W = blockDim.x >> 6;
     const int PG = A0.Pmax + A0.Q + 1;
     // LDS / partial rows are sized for the largest scene; this block's scene may be smaller

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void sample_intervals_kernel(float* __restrict
                                                                float max_jitter, int64_t n_rays, int m, int n,
                                                                float minval, float maxval, int waves) {
     extern __shared__ float sf_lds[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     float* cw = sf_lds + (int64_t)wave * (2 * (m + 1) + n);
     float* te = cw + (m + 1);
     float* cs = te + (m + 1);
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void max_dilate_kernel(float* __restrict__ t_o
                                                          int64_t n_rays, int m, float dilation, float minval,
                                                          float maxval, int renormalize, int waves) {
     extern __shared__ float sf_lds[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int ne = 3 * m + 1;
     float* ts = sf_lds + (int64_t)wave * (5 * m + 2);
     float* ps = ts + (m + 1);

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(BLOCK) void stroke_fwd_kernel(
     const float* __restrict__ viewdir, const float* __restrict__ sp, const float* __restrict__ cp, int64_t N,
     int64_t S, int64_t spr, int P, int Q, uint32_t flags, float delta, int laplace, int isr, int64_t ppb) {
     constexpr int NB = num_base_params(BASE);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t k = (int64_t)blockIdx.y * 64 + lane;
     const bool kv = k < S;
     const int64_t kk = kv ? k : S - 1;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void stroke_bwd_kernel(
     int64_t ppb) {
     constexpr int NB = num_base_params(BASE);
     constexpr int NQ = num_color_params(CID);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int64_t k = (int64_t)blockIdx.y * 64 + lane;
     const bool kv = k < S;
     const int64_t kk = kv ? k : S - 1;

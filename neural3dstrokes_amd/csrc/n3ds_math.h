@@ -42,6 +42,11 @@ __device__ __forceinline__ float safe_len(V3 v) { float s = dot(v, v); return s 
 // ---------------------------------------------------------------------------
 // wavefront helpers (64 lanes on CDNA)
 // ---------------------------------------------------------------------------
+// lane j's value (a scalar register: uniform control flow only, every lane active)
+__device__ __forceinline__ int lane_i(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float lane_f(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -642,7 +642,7 @@ __device__ __forceinline__ int class_sort_hits(const WaveMem& w, int hc, int hcl
             xo = before_pairs + incl - cnt;
         }
         before_hits += __popcll(m);
-        before_pairs += __shfl(incl, 63, 64);
+        before_pairs += lane_i(incl, 63);
     }
     if (hcls >= 0) {
         w.xo[rank] = xo;
@@ -841,7 +841,7 @@ __device__ __forceinline__ int append_hits(const RenderArgs& A, const RayConst& 
         // class | provisional (curve still to refine) << 2 | passing pieces << 16
         w.ccls[rank] = cls | (curve ? 4 | (pmask << 16) : 0);
     }
-    tot = __shfl(incl, 63, 64);
+    tot = lane_i(incl, 63);
     wave_sync();
     return __popcll(mask);
 }
@@ -880,11 +880,6 @@ __device__ __forceinline__ WindowHits<CID> window_hits(const RenderArgs& A, cons
         h.cu = stroke_color_uniform<CID>(A.cparams + (int64_t)h.k * A.Q, vd);
     }
     return h;
-}
-
-__device__ __forceinline__ int lane_i(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
-__device__ __forceinline__ float lane_f(float v, int j) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
 // Number of pending hits (in compositing order) whose stroke lies in a chunk
@@ -931,7 +926,7 @@ __device__ __forceinline__ int refine_pending(const RenderArgs& A, const RayCons
             w.coff[r] = pairs + incl - cnt;
         }
         out += __popcll(km);
-        pairs += __shfl(incl, 63, 64);
+        pairs += lane_i(incl, 63);
         wave_sync();
     }
     np = pairs;
@@ -1125,7 +1120,7 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
                 pm = (st & 1) ? ((unsigned)st >> 16) : 0u;
                 np = __popc(pm);
                 incl = wave_incl_scan(np, lane);
-                tot = __shfl(incl, 63, 64);
+                tot = lane_i(incl, 63);
             }
             int done = 0;
             do {
@@ -1195,7 +1190,7 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_kernel(RenderArgs A0,
             const unsigned long long mask = __ballot(hit);
             const int incl = wave_incl_scan(hit ? cnt : 0, lane);
             if (hit) slots[slot * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = pack_hit(lane, i0, cnt, cls);
-            const int tot = __shfl(incl, 63, 64);
+            const int tot = lane_i(incl, 63);
             if (lane == 0) counts[slot] = __popcll(mask) | (tot << 8);
         }
         wave_sync();
@@ -1598,7 +1593,7 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_grid_kernel(RenderArg
         chits[lane] = 0;
         cpairs[lane] = 0;
     }
-    const int total = __shfl(pincl, 63, 64);
+    const int total = lane_i(pincl, 63);
     if (lane == 0) cfirst[nch] = total;
     wave_sync();
     const int64_t slot0 = ray * A.nch_stride;
@@ -1675,7 +1670,7 @@ __global__ __launch_bounds__(64 * CWAVES) void render_cull_grid_kernel(RenderArg
                 pm = (st & 1) ? ((unsigned)st >> 16) : 0u;
                 npc = __popc(pm);
                 incl = wave_incl_scan(npc, lane);
-                tot = __shfl(incl, 63, 64);
+                tot = lane_i(incl, 63);
             }
             int done = 0;
             do {
@@ -1817,7 +1812,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
             cword = lane < A.nch ? A.ccounts[slot0 + lane] : 0;
             const int ncl = cword & 127;
             const int hincl = wave_incl_scan(ncl, lane);
-            const int H = __shfl(hincl, 63, 64);
+            const int H = lane_i(hincl, 63);
             if (H <= HMAX) {
                 w.xo[lane] = hincl - ncl;
                 wave_sync();
@@ -1840,7 +1835,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
                         w.ccls[hi] = (hw >> 24) & 3;
                         w.coff[hi] = carry + incl - cnt;
                     }
-                    carry += __shfl(incl, 63, 64);
+                    carry += lane_i(incl, 63);
                 }
                 wave_sync();
                 nh = H;
@@ -2042,7 +2037,7 @@ __device__ __forceinline__ void render_fwd_body(const RenderArgs& A0, float* __r
         const float trans = expf(-excl);
         const float al = (A.opaque && lane + 64 * s == A.n - 1) ? 1.f : 1.f - expf(-sd[s]);
         const float wt = valid[s] ? al * trans : 0.f;
-        carry += __shfl(v, 63, 64);
+        carry += lane_f(v, 63);
         acc += wt;
         r0 += wt * rgbn[s][0];
         r1 += wt * rgbn[s][1];
@@ -2119,7 +2114,7 @@ __global__ __launch_bounds__(64 * XWAVES) void render_extras_kernel(const float*
         const float trans = expf(-(carry + v - sd));
         const float al = (opaque && i == n - 1) ? 1.f : 1.f - expf(-sd);
         const float w = valid ? al * trans : 0.f;
-        carry += __shfl(v, 63, 64);
+        carry += lane_f(v, 63);
         acc += w;
         if (valid) lsum += w * logf(0.5f * (ta + tb));
         // fencepost i + 1: the integrated weight of samples 0..i (clamped to 1) at depth td[i + 1]
@@ -2130,7 +2125,7 @@ __global__ __launch_bounds__(64 * XWAVES) void render_extras_kernel(const float*
             if (lane >= off) cw += u;
         }
         const float cwi = fminf(wcarry + cw, 1.f);
-        wcarry += __shfl(cw, 63, 64);
+        wcarry += lane_f(cw, 63);
         if (valid) {
 #pragma unroll
             for (int q = 0; q < 3; q++) {
@@ -2556,7 +2551,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
             trv[s] = expf(-excl);
             const float al = (A.opaque && lane + 64 * s == A.n - 1) ? 1.f : 1.f - expf(-sdl[s]);
             wv[s] = valid[s] ? al * trv[s] : 0.f;
-            carry += __shfl(v, 63, 64);
+            carry += lane_f(v, 63);
             acc += wv[s];
             dsum += wv[s] * ts[s];
         }
@@ -2591,7 +2586,7 @@ __global__ __launch_bounds__(64 * BWAVES) void render_bwd_kernel(RenderArgs A0, 
             // transmittance past the sample (zero past an opaque last interval)
             const float tnext = (A.opaque && lane + 64 * s == A.n - 1) ? 0.f : trv[s] * expf(-sdl[s]);
             gsig[s] = delta_s[s] * (tnext * gw[s] - excl);
-            suf_in += __shfl(v, 0, 64);
+            suf_in += lane_f(v, 0);
         }
         // per-sample compositing adjoint inputs
         float gD[NS], O[NS], iOs[NS], Tp[NS], D2[NS];

@@ -438,6 +438,28 @@ __device__ __forceinline__ bool cull_stroke(const float* __restrict__ rec, V3 o,
         slab_inv(A.x, __builtin_amdgcn_rcpf(B.x), -ex, ex, t0, t1);
         slab_inv(A.y, __builtin_amdgcn_rcpf(B.y), -ey, ey, t0, t1);
         slab_inv(A.z, __builtin_amdgcn_rcpf(B.z), -ez, ez, t0, t1);
+        const int base = ibits(rec[R_BASE]);
+        if (base == OCTAHEDRON || base == TETRAHEDRON) {
+            // face planes: sdf < mu implies |x|+|y|+|z| < ex (octahedron) and
+            // max(|x+y| - z, |x-y| + z) < ex (tetrahedron), ex = the box's padded extent;
+            // the sums get a rounding pad of their own
+            const float c = ex + 4e-7f * (fabsf(A.x) + fabsf(A.y) + fabsf(A.z));
+            const float pa = A.x + A.y - A.z, pb = B.x + B.y - B.z;  // x + y - z
+            const float qa = A.x - A.y + A.z, qb = B.x - B.y + B.z;  // x - y + z
+            const float ra = A.y + A.z - A.x, rb = B.y + B.z - B.x;  // -x + y + z
+            const float sa = A.x + A.y + A.z, sb = B.x + B.y + B.z;  // x + y + z
+            if (base == OCTAHEDRON) {
+                slab_inv(pa, __builtin_amdgcn_rcpf(pb), -c, c, t0, t1);
+                slab_inv(qa, __builtin_amdgcn_rcpf(qb), -c, c, t0, t1);
+                slab_inv(ra, __builtin_amdgcn_rcpf(rb), -c, c, t0, t1);
+                slab_inv(sa, __builtin_amdgcn_rcpf(sb), -c, c, t0, t1);
+            } else {
+                slab_inv(pa, __builtin_amdgcn_rcpf(pb), -1e30f, c, t0, t1);
+                slab_inv(qa, __builtin_amdgcn_rcpf(qb), -1e30f, c, t0, t1);
+                slab_inv(ra, __builtin_amdgcn_rcpf(rb), -1e30f, c, t0, t1);
+                slab_inv(-sa, __builtin_amdgcn_rcpf(-sb), -1e30f, c, t0, t1);  // -x - y - z
+            }
+        }
     }
     if (!(t0 <= t1)) return false;
     pad_interval(t0, t1, ta, tb);
